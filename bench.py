@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the per-pixel back-projection on one MI355X.
+
+Workload: the 4000 x 250-pixel scene over 40001 pulses (RawData 4762 x 40001 complex64, ~1.5 GB,
+2000 point targets + 20 dB noise, Sint ~ U[2500,3500], mask ~ Bernoulli(0.95)), synthetic and
+seeded.  One step = one focusing pass over every pixel with RawData, trajectory and pixel arrays
+already resident in HBM.  Unit of work: pixel-pulse pairs (pp) evaluated, counted on the host from
+the integer inputs.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "sar-backprojection_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+BYTES_PER_PP = 8.0     # one complex64 RawData sample gathered per pixel-pulse pair
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def phase_amp_errors(test, ref):
+    """Wrapped phase error (compare_phase.py:50-56, 91) over |a| > 1e-6 and over |a| > 1e-5*max
+    (choerence_measure.py:20, 195-197), and relative amplitude errors."""
+    a_t, a_r = np.abs(test), np.abs(ref)
+    out = {}
+    for name, thr in (("abs1e-6", 1e-6), ("rel1e-5max", 1e-5 * float(a_r.max()))):
+        m = (a_t > thr) & (a_r > thr)
+        d = np.angle(np.exp(1j * (np.angle(test[m]) - np.angle(ref[m]))))
+        out[f"phase_rms_rad_{name}"] = float(np.sqrt(np.mean(d ** 2))) if m.any() else None
+        out[f"phase_max_rad_{name}"] = float(np.abs(d).max()) if m.any() else None
+    m = a_r > 1e-5 * float(a_r.max())
+    out["amp_rel_rmse"] = float(np.sqrt(np.sum((a_t - a_r) ** 2) / np.sum(a_r ** 2)))
+    out["amp_rel_max_rel1e-5max"] = float((np.abs(a_t[m] - a_r[m]) / a_r[m]).max()) if m.any() else None
+    return out
+
+
+def time_precision(scene, precision, steps, warmup):
+    for _ in range(warmup):
+        scene.focus(precision)
+    walls, gpus = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        g = scene.focus(precision)
+        walls.append(time.perf_counter() - t0)
+        gpus.append(g)
+    return float(np.mean(walls)), float(np.mean(gpus)) / 1e3
+
+
+def cpu_baseline(sc, strip_cols, gpu_strip):
+    """Oracle (C restatement) on a bounded strip of the same workload, all host cores, compared
+    against the GPU result of the same pixels."""
+    from oracle import oracle_lib as O
+    x0 = sc.shape[0] // 2 - strip_cols // 2
+    st = sc.sub(x0, x0 + strip_cols)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = min(cores, 16)
+    pp = O.count_pp(*[np.asarray(a).flatten(order="F") for a in (st.mask, st.index, st.sint)], st.Naz)
+    t0 = time.perf_counter()
+    ref, rcnt = O.focus(*st.args(), nthreads=cores)
+    dt = time.perf_counter() - t0
+    got = gpu_strip(x0, x0 + strip_cols)
+    nz = rcnt.ravel() > 0
+    rel = np.abs(got.ravel()[nz] - ref.ravel()[nz]) / np.abs(ref.ravel()[nz])
+    zeros_ok = bool(np.all(got.ravel()[~nz] == 0))
+    return {"value": pp / dt, "unit": "pixel·pulses/s", "cores": int(cores), "kind": "port",
+            "sample": f"{strip_cols}x{sc.shape[1]} px strip (x {x0}..{x0 + strip_cols - 1}) of the same "
+                      f"workload, {pp} pp, {dt:.2f} s, C restatement of process_point with OpenMP"}, \
+        {"strip_f64_max_rel_err": float(rel.max()) if nz.any() else 0.0, "strip_zero_pixels_exact": zeros_ok}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--strip", type=int, default=256, help="pixel columns of the CPU baseline strip")
+    ap.add_argument("--precisions", default="f64,split")
+    ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pixel sharding across GPUs "
+              "is described, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from sarbp import synth
+    from sarbp.core import Dataset
+
+    t0 = time.perf_counter()
+    if args.small:
+        sc = synth.full_scene(seed=1, nx=512, ny=250, naz=5121, ntarget=200)
+    else:
+        sc = synth.full_scene(seed=1)
+    log(f"scene {sc.name} generated in {time.perf_counter() - t0:.1f} s")
+    ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    xf, yf, zf, mf, inf, sf = sc.flat()
+    scene = ds.scene(xf, yf, zf, mf, inf, sf)
+    pp = scene.count_pp()
+    log(f"dataset resident ({ds.device_bytes / 1e9:.2f} GB), {scene.n} pixels, {pp} pp")
+
+    res = {}
+    outs = {}
+    for prec in [p for p in args.precisions.split(",") if p]:
+        try:
+            wall, gpu_s = time_precision(scene, prec, args.steps, args.warmup)
+        except ValueError as e:
+            log(f"{prec}: unavailable ({e})")
+            continue
+        res[prec] = {"wall_s": wall, "kernel_s": gpu_s, "pp_per_s": pp / wall}
+        outs[prec] = scene.result()
+        log(f"{prec}: {wall * 1e3:.2f} ms/step (kernel {gpu_s * 1e3:.2f} ms), {pp / wall:.4e} pp/s")
+
+    head = "split" if "split" in res else "f64"
+    extra = {"pp_per_step": pp, "pixels": int(scene.n)}
+    for p, r in res.items():
+        extra[f"{p}_pp_per_s"] = r["pp_per_s"]
+        extra[f"{p}_kernel_ms"] = r["kernel_s"] * 1e3
+    if "split" in res and "f64" in res:
+        extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
+        extra["split_vs_f64_errors"] = phase_amp_errors(outs["split"], outs["f64"])
+
+    cpu = None
+    if not args.no_cpu:
+        full_f64 = outs.get("f64")
+        shape = sc.shape
+
+        def gpu_strip(a, b):
+            if full_f64 is None:
+                return None
+            return full_f64.reshape(shape, order="F")[a:b]
+
+        cpu, parity = cpu_baseline(sc, args.strip, gpu_strip)
+        extra.update(parity)
+        log(f"cpu baseline {cpu['value']:.4e} pp/s on {cpu['cores']} cores; strip parity {parity}")
+
+    r = res[head]
+    kernel_s = r["kernel_s"]
+    achieved = BYTES_PER_PP * pp / kernel_s / 1e9
+    line = {
+        "metric": "pixel·pulses/s split-FP32 vs FP64 at ≤1e-3 rad phase err; p50/p99 request latency",
+        "value": r["pp_per_s"],
+        "unit": "pixel·pulses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["wall_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 split" if head == "split" else "f64",
+        "data": "synthetic (seeded point targets + 20 dB noise)",
+        "config": {"workload": f"{sc.shape[0]}x{sc.shape[1]} px x {sc.Naz} pulses, Nra {sc.Nra}, "
+                               f"Sint U[2500,3500], mask 0.95",
+                   "precision": head, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    scene.close()
+    ds.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,120 @@
+/*
+ * sarbp.h -- C ABI of the MI355X (gfx950) SAR back-projection library, libsarbp.so.
+ *
+ * Plain pointers and sizes only.  No exceptions cross this boundary: every entry point returns
+ * an int status (SARBP_OK == 0, negative on failure); sarbp_last_error() gives a message for the
+ * calling thread's last failure.  The caller owns every host buffer; the library owns the device
+ * buffers behind the opaque handles.  Calls on one handle must not overlap; different handles may
+ * be used from different threads.
+ *
+ * Layouts (identical to the reference's in-memory arrays):
+ *   RawData   complex64, column-major (Nra, Naz): sample (r, a) at element r + Nra*a, each
+ *             element two float32 (re, im) -- sar_focuser_puntual.py:480-491, main_gpu.py:375-381.
+ *   pixels    F-order flattened DEM grid: x/y/z float64, mask uint8, index/sint int32
+ *             -- sar_focuser_puntual.py:195-201.
+ *   output    complex128 (re, im float64 pairs) per pixel, same flat order; reshape (dimx, dimy)
+ *             in F order -- sar_focuser_puntual.py:306.
+ */
+#ifndef SARBP_H
+#define SARBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SARBP_ABI_VERSION 1
+
+/* status codes */
+#define SARBP_OK 0
+#define SARBP_EINVAL -1    /* invalid argument (null pointer, bad size, mismatched shapes) */
+#define SARBP_ENOMEM -2    /* device or pinned-host allocation failed */
+#define SARBP_ELAUNCH -3   /* kernel launch failed */
+#define SARBP_EDEVICE -4   /* HIP runtime error (device lost, copy failed, ...) */
+#define SARBP_ENODEV -5    /* no HIP device */
+#define SARBP_EBUDGET -6   /* request exceeds the device-memory admission budget */
+#define SARBP_ESTATE -7    /* call not valid in the object's current state */
+#define SARBP_EAGAIN -8    /* not finished yet (poll) */
+
+/* arithmetic of a focusing pass */
+#define SARBP_PREC_F64 0   /* plain FP64, reference operation order (parity path) */
+#define SARBP_PREC_SPLIT 1 /* FP64 per-tile reference range + FP32 offsets, Kahan accumulation */
+
+typedef struct sarbp_dataset sarbp_dataset; /* device-resident RawData + trajectory + scalars */
+typedef struct sarbp_scene sarbp_scene;     /* device-resident pixel set + its output buffers */
+
+int sarbp_abi_version(void);
+const char *sarbp_strerror(int status);
+/* Message of the calling thread's last failed call ("" if none). */
+const char *sarbp_last_error(void);
+/* Number of visible HIP devices. */
+int sarbp_device_count(int *count);
+
+/*
+ * Upload RawData and the trajectory once; they stay in HBM until sarbp_dataset_destroy.
+ * Replaces the per-call host transpose + H2D copies of FocalizzatoreBpCuPy
+ * (gpu_small_dataset.py:149-159, 262-275) and the per-batch raw window staging of
+ * SARSuperOptimizedFocuser.process_dataset (main_gpu.py:465-515).
+ * raw: complex64 column-major (nra, naz); tx/ty/tz: float64[naz];
+ * near_range, dra, lambda_val: NearRangeDbl / DeltaRangeDbl / LambdaDbl (sar_focuser_puntual.py:505-507).
+ */
+int sarbp_dataset_create(const void *raw_c64, int64_t nra, int64_t naz, const double *tx,
+                         const double *ty, const double *tz, double near_range, double dra,
+                         double lambda_val, sarbp_dataset **out);
+/* Same, uploading RawData in pulse blocks from a caller callback (for memmapped files larger
+ * than host RAM, main_gpu.py:372-385): fill(user, a0, a1, dst) writes pulses [a0, a1) as
+ * complex64 column-major into dst and returns 0, or non-zero to abort. */
+typedef int (*sarbp_fill_fn)(void *user, int64_t a0, int64_t a1, void *dst);
+int sarbp_dataset_create_streamed(sarbp_fill_fn fill, void *user, int64_t nra, int64_t naz,
+                                  const double *tx, const double *ty, const double *tz,
+                                  double near_range, double dra, double lambda_val,
+                                  sarbp_dataset **out);
+int sarbp_dataset_destroy(sarbp_dataset *ds);
+/* Device bytes held by the dataset. */
+int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes);
+
+/*
+ * Upload a pixel set (F-order flattened DEM grid) for repeated focusing.  Pixels whose sint is
+ * -9999 are masked, as FocalizzatoreBpPx does (sar_focuser_puntual.py:208-211).
+ */
+int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                       const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
+                       sarbp_scene **out);
+int sarbp_scene_destroy(sarbp_scene *sc);
+
+/*
+ * Focus every pixel of the scene: for each pixel the mean over its aperture of
+ * R^2 * RawData[r, a] * exp(-j*phi), with the gates, range-bin rule and phase of
+ * process_point (sar_focuser_puntual.py:21-155).  Results stay on the device until
+ * sarbp_scene_download.  gpu_ms (nullable) receives the kernel time measured with HIP events
+ * on the library's stream.
+ * SARBP_PREC_F64 is the drop-in for process_point_kernel (gpu_small_dataset.py:9-108,
+ * main_gpu.py:60-155) with the Numba path's semantics.
+ */
+int sarbp_scene_focus(sarbp_dataset *ds, sarbp_scene *sc, int precision, float *gpu_ms);
+/* Copy results to the host: out_c128 (2*n float64, nullable), ncontrib (n int32, nullable). */
+int sarbp_scene_download(sarbp_scene *sc, double *out_c128, int32_t *ncontrib);
+
+/*
+ * One-shot: scene_create + scene_focus + scene_download + scene_destroy.  The C-ABI behind
+ * FocalizzatoreBpPx (sar_focuser_puntual.py:160) and FocalizzatoreBpCuPy(_Batched)
+ * (gpu_small_dataset.py:113, 229).
+ */
+int sarbp_focus(sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
+                int precision, double *out_c128, int32_t *ncontrib);
+
+/*
+ * Pixel-pulse pairs evaluated by a focusing pass (host-side, integers only): pixels with
+ * mask == 1 and sint != -9999, pulses 0 < index + j - ceil(|sint|/2) < naz - 1 for
+ * j < 2*ceil(sint/2) + 1.  The unit of the throughput metric.
+ */
+int64_t sarbp_count_pp(const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
+                       int64_t naz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SARBP_H */

@@ -1,0 +1,341 @@
+// Host side of the C ABI declared in include/sarbp.h: device-resident datasets and pixel sets,
+// focusing passes, error reporting.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "sarbp_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class T>
+int dev_alloc(T **p, size_t count, int64_t &acct, const char *what) {
+  *p = nullptr;
+  if (count == 0) return SARBP_OK;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return sarbp_fail(SARBP_ENOMEM, std::string("hipMalloc failed for ") + what + ": " + hipGetErrorString(e));
+  }
+  acct += (int64_t)(count * sizeof(T));
+  return SARBP_OK;
+}
+
+void dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+int h2d(void *dst, const void *src, size_t bytes, hipStream_t st, const char *what) {
+  if (bytes == 0) return SARBP_OK;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return sarbp_hip_fail(e, what);
+  return SARBP_OK;
+}
+
+int check_device() {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0) {
+    (void)hipGetLastError();
+    return sarbp_fail(SARBP_ENODEV, "no HIP device visible");
+  }
+  return SARBP_OK;
+}
+
+int upload_traj(sarbp_dataset *ds, const double *tx, const double *ty, const double *tz) {
+  std::vector<double4> t((size_t)ds->naz);
+  for (int64_t a = 0; a < ds->naz; ++a) t[(size_t)a] = make_double4(tx[a], ty[a], tz[a], 0.0);
+  return h2d(ds->traj, t.data(), t.size() * sizeof(double4), ds->stream, "trajectory upload");
+}
+
+int dataset_init(int64_t nra, int64_t naz, const double *tx, const double *ty, const double *tz,
+                 double near_range, double dra, double lambda_val, sarbp_dataset **out) {
+  if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
+  *out = nullptr;
+  if (nra < 2 || naz < 3) return sarbp_fail(SARBP_EINVAL, "RawData must have nra >= 2 and naz >= 3");
+  if (nra > (int64_t)0x7fffffff || naz > (int64_t)0x7fffffff)
+    return sarbp_fail(SARBP_EINVAL, "nra and naz must each fit in int32");
+  if (!tx || !ty || !tz) return sarbp_fail(SARBP_EINVAL, "trajectory pointer is null");
+  if (!(dra != 0.0) || !std::isfinite(dra) || !std::isfinite(near_range) || !(lambda_val != 0.0) ||
+      !std::isfinite(lambda_val))
+    return sarbp_fail(SARBP_EINVAL, "near_range, dra and lambda must be finite, dra and lambda non-zero");
+  int rc = check_device();
+  if (rc) return rc;
+  sarbp_dataset *ds = new (std::nothrow) sarbp_dataset();
+  if (!ds) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
+  ds->nra = nra;
+  ds->naz = naz;
+  ds->near_range = near_range;
+  ds->dra = dra;
+  ds->lambda_val = lambda_val;
+  hipError_t e = hipGetDevice(&ds->device);
+  if (e == hipSuccess) e = hipStreamCreate(&ds->stream);
+  if (e == hipSuccess) e = hipEventCreate(&ds->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ds->ev1);
+  if (e != hipSuccess) {
+    rc = sarbp_hip_fail(e, "stream/event creation");
+    sarbp_dataset_destroy(ds);
+    return rc;
+  }
+  rc = dev_alloc(&ds->raw, (size_t)nra * (size_t)naz, ds->bytes, "RawData");
+  if (!rc) rc = dev_alloc(&ds->traj, (size_t)naz, ds->bytes, "trajectory");
+  if (!rc) rc = upload_traj(ds, tx, ty, tz);
+  if (rc) {
+    sarbp_dataset_destroy(ds);
+    return rc;
+  }
+  *out = ds;
+  return SARBP_OK;
+}
+}  // namespace
+
+int sarbp_fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int sarbp_hip_fail(hipError_t e, const char *what) {
+  g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+  (void)hipGetLastError();
+  return SARBP_EDEVICE;
+}
+
+BpGeom sarbp_dataset::geom() const {
+  BpGeom g;
+  g.raw = raw;
+  g.traj = traj;
+  g.nra = nra;
+  g.naz = naz;
+  g.near_range = near_range;
+  g.dra = dra;
+  g.inv_dra = 1.0 / dra;
+  // Same expressions and evaluation order as sar_focuser_puntual.py:40 and :116.
+  const double pi = SARBP_PI32_F64;
+  g.kphase = -4.0 * pi / lambda_val;
+  g.beta = 4.0 * pi * SARBP_ALFA_F64 / (SARBP_C_F64 * SARBP_C_F64);
+  return g;
+}
+
+extern "C" {
+
+int sarbp_abi_version(void) { return SARBP_ABI_VERSION; }
+
+const char *sarbp_strerror(int status) {
+  switch (status) {
+    case SARBP_OK: return "ok";
+    case SARBP_EINVAL: return "invalid argument";
+    case SARBP_ENOMEM: return "out of memory";
+    case SARBP_ELAUNCH: return "kernel launch failed";
+    case SARBP_EDEVICE: return "HIP runtime error";
+    case SARBP_ENODEV: return "no HIP device";
+    case SARBP_EBUDGET: return "device-memory budget exceeded";
+    case SARBP_ESTATE: return "invalid state";
+    case SARBP_EAGAIN: return "not finished";
+    default: return "unknown status";
+  }
+}
+
+const char *sarbp_last_error(void) { return g_last_error.c_str(); }
+
+int sarbp_device_count(int *count) {
+  if (!count) return sarbp_fail(SARBP_EINVAL, "count is null");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    (void)hipGetLastError();
+  }
+  return SARBP_OK;
+}
+
+int sarbp_dataset_create(const void *raw_c64, int64_t nra, int64_t naz, const double *tx, const double *ty,
+                         const double *tz, double near_range, double dra, double lambda_val,
+                         sarbp_dataset **out) {
+  if (!raw_c64) return sarbp_fail(SARBP_EINVAL, "raw is null");
+  int rc = dataset_init(nra, naz, tx, ty, tz, near_range, dra, lambda_val, out);
+  if (rc) return rc;
+  sarbp_dataset *ds = *out;
+  rc = h2d(ds->raw, raw_c64, (size_t)nra * (size_t)naz * sizeof(float2), ds->stream, "RawData upload");
+  if (rc) {
+    sarbp_dataset_destroy(ds);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+int sarbp_dataset_create_streamed(sarbp_fill_fn fill, void *user, int64_t nra, int64_t naz, const double *tx,
+                                  const double *ty, const double *tz, double near_range, double dra,
+                                  double lambda_val, sarbp_dataset **out) {
+  if (!fill) return sarbp_fail(SARBP_EINVAL, "fill callback is null");
+  int rc = dataset_init(nra, naz, tx, ty, tz, near_range, dra, lambda_val, out);
+  if (rc) return rc;
+  sarbp_dataset *ds = *out;
+  // Pinned staging in ~64 MiB pulse blocks, double-buffered so the callback overlaps the copy.
+  const int64_t col = nra * (int64_t)sizeof(float2);
+  int64_t blk = (int64_t)(64 << 20) / col;
+  if (blk < 1) blk = 1;
+  void *stage[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+  hipError_t e = hipHostMalloc(&stage[0], (size_t)(blk * col), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipHostMalloc(&stage[1], (size_t)(blk * col), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipEventCreate(&done[0]);
+  if (e == hipSuccess) e = hipEventCreate(&done[1]);
+  if (e != hipSuccess) rc = sarbp_hip_fail(e, "staging allocation");
+  int k = 0;
+  for (int64_t a0 = 0; !rc && a0 < naz; a0 += blk, k ^= 1) {
+    const int64_t a1 = a0 + blk < naz ? a0 + blk : naz;
+    e = hipEventSynchronize(done[k]);
+    if (e != hipSuccess) { rc = sarbp_hip_fail(e, "staging wait"); break; }
+    if (fill(user, a0, a1, stage[k]) != 0) { rc = sarbp_fail(SARBP_EINVAL, "fill callback failed"); break; }
+    e = hipMemcpyAsync(ds->raw + nra * a0, stage[k], (size_t)((a1 - a0) * col), hipMemcpyHostToDevice,
+                       ds->stream);
+    if (e == hipSuccess) e = hipEventRecord(done[k], ds->stream);
+    if (e != hipSuccess) rc = sarbp_hip_fail(e, "RawData block upload");
+  }
+  e = hipStreamSynchronize(ds->stream);
+  if (!rc && e != hipSuccess) rc = sarbp_hip_fail(e, "RawData upload");
+  for (int i = 0; i < 2; ++i) {
+    if (stage[i]) (void)hipHostFree(stage[i]);
+    if (done[i]) (void)hipEventDestroy(done[i]);
+  }
+  if (rc) {
+    sarbp_dataset_destroy(ds);
+    *out = nullptr;
+  }
+  return rc;
+}
+
+int sarbp_dataset_destroy(sarbp_dataset *ds) {
+  if (!ds) return SARBP_OK;
+  if (ds->stream) (void)hipStreamSynchronize(ds->stream);
+  dev_free(ds->raw);
+  dev_free(ds->traj);
+  if (ds->ev0) (void)hipEventDestroy(ds->ev0);
+  if (ds->ev1) (void)hipEventDestroy(ds->ev1);
+  if (ds->stream) (void)hipStreamDestroy(ds->stream);
+  delete ds;
+  return SARBP_OK;
+}
+
+int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes) {
+  if (!ds || !bytes) return sarbp_fail(SARBP_EINVAL, "null argument");
+  *bytes = ds->bytes;
+  return SARBP_OK;
+}
+
+int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, const double *z, const uint8_t *mask,
+                       const int32_t *index, const int32_t *sint, int64_t n, sarbp_scene **out) {
+  if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
+  *out = nullptr;
+  if (!ds) return sarbp_fail(SARBP_EINVAL, "dataset is null");
+  if (n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
+  if (n > 0 && (!x || !y || !z || !mask || !index || !sint))
+    return sarbp_fail(SARBP_EINVAL, "pixel array pointer is null");
+  sarbp_scene *sc = new (std::nothrow) sarbp_scene();
+  if (!sc) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
+  sc->ds = ds;
+  sc->n = n;
+  const size_t un = (size_t)n;
+  int rc = dev_alloc(&sc->x, un, sc->bytes, "x");
+  if (!rc) rc = dev_alloc(&sc->y, un, sc->bytes, "y");
+  if (!rc) rc = dev_alloc(&sc->z, un, sc->bytes, "z");
+  if (!rc) rc = dev_alloc(&sc->mask, un, sc->bytes, "mask");
+  if (!rc) rc = dev_alloc(&sc->index, un, sc->bytes, "index");
+  if (!rc) rc = dev_alloc(&sc->sint, un, sc->bytes, "sint");
+  if (!rc) rc = dev_alloc(&sc->out, un, sc->bytes, "output");
+  if (!rc) rc = dev_alloc(&sc->ncnt, un, sc->bytes, "counts");
+  if (!rc) rc = h2d(sc->x, x, un * 8, ds->stream, "x upload");
+  if (!rc) rc = h2d(sc->y, y, un * 8, ds->stream, "y upload");
+  if (!rc) rc = h2d(sc->z, z, un * 8, ds->stream, "z upload");
+  if (!rc) rc = h2d(sc->mask, mask, un, ds->stream, "mask upload");
+  if (!rc) rc = h2d(sc->index, index, un * 4, ds->stream, "index upload");
+  if (!rc) rc = h2d(sc->sint, sint, un * 4, ds->stream, "sint upload");
+  if (rc) {
+    sarbp_scene_destroy(sc);
+    return rc;
+  }
+  *out = sc;
+  return SARBP_OK;
+}
+
+int sarbp_scene_destroy(sarbp_scene *sc) {
+  if (!sc) return SARBP_OK;
+  if (sc->ds && sc->ds->stream) (void)hipStreamSynchronize(sc->ds->stream);
+  dev_free(sc->x);
+  dev_free(sc->y);
+  dev_free(sc->z);
+  dev_free(sc->mask);
+  dev_free(sc->index);
+  dev_free(sc->sint);
+  dev_free(sc->out);
+  dev_free(sc->ncnt);
+  delete sc;
+  return SARBP_OK;
+}
+
+int sarbp_scene_focus(sarbp_dataset *ds, sarbp_scene *sc, int precision, float *gpu_ms) {
+  if (!ds || !sc) return sarbp_fail(SARBP_EINVAL, "null handle");
+  if (sc->ds != ds) return sarbp_fail(SARBP_EINVAL, "scene belongs to another dataset");
+  if (precision != SARBP_PREC_F64 && precision != SARBP_PREC_SPLIT)
+    return sarbp_fail(SARBP_EINVAL, "unknown precision");
+  hipError_t e = hipEventRecord(ds->ev0, ds->stream);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
+  if (precision == SARBP_PREC_SPLIT) return sarbp_fail(SARBP_EINVAL, "split precision is not available yet");
+  int rc = sarbp_launch_f64(ds, sc, ds->stream);
+  if (rc) return rc;
+  e = hipEventRecord(ds->ev1, ds->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ds->ev1);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "focus kernel");
+  if (gpu_ms) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, ds->ev0, ds->ev1);
+    if (e != hipSuccess) return sarbp_hip_fail(e, "event timing");
+    *gpu_ms = ms;
+  }
+  return SARBP_OK;
+}
+
+int sarbp_scene_download(sarbp_scene *sc, double *out_c128, int32_t *ncontrib) {
+  if (!sc) return sarbp_fail(SARBP_EINVAL, "null handle");
+  hipStream_t st = sc->ds->stream;
+  hipError_t e = hipSuccess;
+  if (out_c128 && sc->n)
+    e = hipMemcpyAsync(out_c128, sc->out, (size_t)sc->n * sizeof(double2), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && ncontrib && sc->n)
+    e = hipMemcpyAsync(ncontrib, sc->ncnt, (size_t)sc->n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "result download");
+  return SARBP_OK;
+}
+
+int sarbp_focus(sarbp_dataset *ds, const double *x, const double *y, const double *z, const uint8_t *mask,
+                const int32_t *index, const int32_t *sint, int64_t n, int precision, double *out_c128,
+                int32_t *ncontrib) {
+  sarbp_scene *sc = nullptr;
+  int rc = sarbp_scene_create(ds, x, y, z, mask, index, sint, n, &sc);
+  if (rc) return rc;
+  rc = sarbp_scene_focus(ds, sc, precision, nullptr);
+  if (!rc) rc = sarbp_scene_download(sc, out_c128, ncontrib);
+  sarbp_scene_destroy(sc);
+  return rc;
+}
+
+int64_t sarbp_count_pp(const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n, int64_t naz) {
+  if (!mask || !index || !sint || n <= 0) return 0;
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (mask[i] != 1 || sint[i] == -9999) continue;
+    int64_t lo, hi;
+    sarbp_pulse_range(index[i], sint[i], naz, lo, hi);
+    tot += hi - lo;
+  }
+  return tot;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal declarations shared by the HIP translation units of libsarbp.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "sarbp.h"
+
+// Constants of process_point (sar_focuser_puntual.py:35-40).  PI is float32(pi) held in a double,
+// as the reference does (it copies IDL's single-precision !PI); C is the reference's rounded value.
+#define SARBP_C_F64 2.9979246e8
+#define SARBP_ALFA_F64 3.335987619777693e11
+#define SARBP_PI32_F64 3.141592741012573242
+
+// Geometry + RawData of one dataset as the kernels see it.
+struct BpGeom {
+  const float2 *raw;    // column-major (nra, naz): sample (r, a) at raw[r + nra*a]
+  const double4 *traj;  // per pulse (tx, ty, tz, 0)
+  int64_t nra;
+  int64_t naz;
+  double near_range;
+  double dra;
+  double inv_dra;       // 1/dra, used only where the result is re-checked exactly
+  double kphase;        // (-4*PI32)/lambda        (sar_focuser_puntual.py:116)
+  double beta;          // ((4*PI32)*ALFA)/(C*C)   (sar_focuser_puntual.py:40)
+};
+
+// Pixel set in device memory (F-order flattened DEM grid).
+struct PixelsView {
+  const double *x;
+  const double *y;
+  const double *z;
+  const uint8_t *mask;
+  const int32_t *index;
+  const int32_t *sint;
+  int64_t n;
+};
+
+struct sarbp_dataset {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t nra = 0, naz = 0;
+  double near_range = 0, dra = 0, lambda_val = 0;
+  float2 *raw = nullptr;
+  double4 *traj = nullptr;
+  int64_t bytes = 0;
+  BpGeom geom() const;
+};
+
+struct sarbp_scene {
+  sarbp_dataset *ds = nullptr;
+  int64_t n = 0;
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  uint8_t *mask = nullptr;
+  int32_t *index = nullptr, *sint = nullptr;
+  double2 *out = nullptr;
+  int32_t *ncnt = nullptr;
+  int64_t bytes = 0;
+  PixelsView view() const { return PixelsView{x, y, z, mask, index, sint, n}; }
+};
+
+// Aperture of one pixel (sar_focuser_puntual.py:57-66): J = 2*ceil(s/2) + 1 pulses starting at
+// index - ceil(|s|/2); integer forms of the reference's float ceil on an int32.
+__host__ __device__ inline int64_t sarbp_ceil_half(int64_t s) { return s >= 0 ? (s + 1) / 2 : s / 2; }
+__host__ __device__ inline void sarbp_aperture(int32_t index, int32_t sint, int64_t &a0, int64_t &J) {
+  int64_t s = sint;
+  int64_t h = sarbp_ceil_half(s < 0 ? -s : s);
+  J = sarbp_ceil_half(s) * 2 + 1;
+  a0 = (int64_t)index - h;
+}
+// Pulses actually used: a in [lo, hi) = [a0, a0+J) intersected with (0, naz-1).
+__host__ __device__ inline void sarbp_pulse_range(int32_t index, int32_t sint, int64_t naz,
+                                                  int64_t &lo, int64_t &hi) {
+  int64_t a0, J;
+  sarbp_aperture(index, sint, a0, J);
+  lo = a0 > 1 ? a0 : 1;
+  int64_t e = a0 + J;
+  hi = e < naz - 1 ? e : naz - 1;
+  if (J <= 0 || hi < lo) hi = lo;
+}
+
+// error plumbing (api.hip)
+int sarbp_fail(int code, const std::string &msg);
+int sarbp_hip_fail(hipError_t e, const char *what);
+
+// kernel launchers (bp_f64.hip, bp_split.hip)
+int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, hipStream_t st);

@@ -1,0 +1,102 @@
+"""ctypes binding of libsarbp.so (the C ABI in include/sarbp.h).
+
+The library is the only compute path: if it is missing, or no HIP device is visible, calls raise
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsarbp.so")
+
+OK = 0
+EINVAL = -1
+ENOMEM = -2
+ELAUNCH = -3
+EDEVICE = -4
+ENODEV = -5
+EBUDGET = -6
+ESTATE = -7
+EAGAIN = -8
+
+PREC_F64 = 0
+PREC_SPLIT = 1
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int32
+_d = ctypes.c_double
+_pp = ctypes.POINTER(ctypes.c_void_p)
+
+FILL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p)
+
+# name -> (restype, argtypes); every symbol include/sarbp.h declares.
+SIGNATURES = {
+    "sarbp_abi_version": (ctypes.c_int, []),
+    "sarbp_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "sarbp_last_error": (ctypes.c_char_p, []),
+    "sarbp_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "sarbp_dataset_create": (ctypes.c_int, [_p, _i64, _i64, _p, _p, _p, _d, _d, _d, _pp]),
+    "sarbp_dataset_create_streamed": (ctypes.c_int, [FILL_FN, _p, _i64, _i64, _p, _p, _p, _d, _d, _d, _pp]),
+    "sarbp_dataset_destroy": (ctypes.c_int, [_p]),
+    "sarbp_dataset_bytes": (ctypes.c_int, [_p, ctypes.POINTER(_i64)]),
+    "sarbp_scene_create": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i64, _pp]),
+    "sarbp_scene_destroy": (ctypes.c_int, [_p]),
+    "sarbp_scene_focus": (ctypes.c_int, [_p, _p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "sarbp_scene_download": (ctypes.c_int, [_p, _p, _p]),
+    "sarbp_focus": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i64, ctypes.c_int, _p, _p]),
+    "sarbp_count_pp": (_i64, [_p, _p, _p, _i64, _i64]),
+}
+
+_lib = None
+
+
+class SarbpError(RuntimeError):
+    """A libsarbp call failed; ``status`` is the negative status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+def load():
+    """Load libsarbp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libsarbp.so not built at {LIB_PATH}; run __graft_entry__.build() "
+                          "(make -C sar-backprojection_amd/csrc)")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int, what: str = "") -> None:
+    if status == OK:
+        return
+    L = load()
+    msg = L.sarbp_last_error().decode(errors="replace") or L.sarbp_strerror(status).decode()
+    text = f"{what}: {msg}" if what else msg
+    if status == EINVAL:
+        raise ValueError(text)
+    if status == ENOMEM:
+        raise MemoryError(text)
+    raise SarbpError(status, text)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    load().sarbp_device_count(ctypes.byref(n))
+    return int(n.value)
+
+
+def require_device() -> None:
+    if device_count() <= 0:
+        raise SarbpError(ENODEV, "no HIP device visible: libsarbp needs an MI355X (gfx950) GPU")

@@ -1,0 +1,107 @@
+"""Drop-in for the focusing entry points of sar_focuser_puntual.py, computed on the MI355X.
+
+``FocalizzatoreBpPx`` and ``process_point`` keep the reference's names, argument order, argument
+meaning and exceptions (sar_focuser_puntual.py:21-27, 160-183); the per-pixel work runs in
+libsarbp's FP64 HIP kernel instead of the Numba loop.
+"""
+from __future__ import annotations
+
+import os
+import weakref
+
+import numpy as np
+
+from . import formats
+from .core import Dataset, precision_code
+
+CHECKPOINT_FILE = "focalizador_px_checkpoint.npz"  # sar_focuser_puntual.py:218
+
+_cache: dict = {}
+
+
+def _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val) -> Dataset:
+    """Device dataset for these arrays, reused while the same host buffers are passed again
+    (process_point is called once per pixel by the reference's loop)."""
+    raw = np.asarray(Raw_Data)
+    key = (raw.__array_interface__["data"][0], raw.shape, raw.strides, str(raw.dtype),
+           tuple(np.asarray(t).__array_interface__["data"][0] for t in (wgsx, wgsy, wgsz)),
+           float(near_range), float(dra), float(lambda_val))
+    ent = _cache.get(key)
+    if ent is not None:
+        ref, ds = ent
+        if ref() is raw or ref() is Raw_Data:
+            return ds
+    ds = Dataset(raw, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
+    _cache.clear()  # keep at most one resident dataset for the compatibility entry points
+    try:
+        ref = weakref.ref(Raw_Data)
+    except TypeError:
+        ref = lambda: Raw_Data  # noqa: E731
+    _cache[key] = (ref, ds)
+    return ds
+
+
+def process_point(xxg1_flat_ii, yyg1_flat_ii, zzg1_flat_ii, mask_flat_ii, index_flat_ii, Sint_flat_ii,
+                  wgsx, wgsy, wgsz, Raw_Data, near_range, dra, lambda_val, Nra, Naz,
+                  ind_line_ra_min=0, index_min=0):
+    """One pixel (sar_focuser_puntual.py:21-155); returns complex128."""
+    if ind_line_ra_min != 0 or index_min != 0:
+        raise NotImplementedError("only ind_line_ra_min = index_min = 0 is supported "
+                                  "(the reference always passes 0, sar_focuser_puntual.py:214-215)")
+    raw = np.asarray(Raw_Data)
+    if raw.ndim != 2 or raw.shape != (int(Nra), int(Naz)):
+        raise ValueError(f"Raw_Data shape {raw.shape} does not match Nra={Nra}, Naz={Naz}")
+    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
+    out = ds.focus_flat(np.array([xxg1_flat_ii], np.float64), np.array([yyg1_flat_ii], np.float64),
+                        np.array([zzg1_flat_ii], np.float64), np.array([mask_flat_ii], np.uint8),
+                        np.array([index_flat_ii], np.int32), np.array([Sint_flat_ii], np.int32))
+    return np.complex128(out[0])
+
+
+def _flatten_inputs(xxg1, yyg1, zzg1, mask, index, Sint):
+    f = lambda a, dt: np.ascontiguousarray(np.asarray(a).astype(dt).flatten(order="F"))
+    xf, yf, zf = f(xxg1, np.float64), f(yyg1, np.float64), f(zzg1, np.float64)
+    mf, inf, sf = f(mask, np.uint8), f(index, np.int32), f(Sint, np.int32)
+    mf[sf == -9999] = 0  # sar_focuser_puntual.py:208-211
+    return xf, yf, zf, mf, inf, sf
+
+
+def FocalizzatoreBpPx(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz, Raw_Data, near_range, dra, lambda_val,
+                      Sint, precision="f64", checkpoint_file=CHECKPOINT_FILE, checkpoint_every=None):
+    """Pixel-by-pixel focuser (sar_focuser_puntual.py:160-330) on the GPU.
+
+    Returns complex128 of shape ``xxg1.shape``.  Inputs are flattened in Fortran order; pixels
+    with Sint == -9999 are masked.  If ``checkpoint_file`` exists with a matching size, the pass
+    resumes at its ``next_idx`` and keeps the saved values before it (sar_focuser_puntual.py:218-234);
+    with ``checkpoint_every`` set, a checkpoint is written after every that many pixels; the file is
+    removed at the end (:322-328).  ``precision='split'`` selects the split-FP32 kernel."""
+    raw = np.asarray(Raw_Data)
+    if raw.ndim != 2:
+        raise ValueError("Raw_Data must be a 2D array")
+    if np.ndim(xxg1) != 2:
+        raise ValueError("xxg1 must be a 2D array")
+    dimx, dimy = np.shape(xxg1)
+    dim = dimx * dimy
+    precision_code(precision)
+    xf, yf, zf, mf, inf, sf = _flatten_inputs(xxg1, yyg1, zzg1, mask, index, Sint)
+    focused = np.zeros(dim, np.complex128)
+    start = 0
+    if checkpoint_file:
+        ck = formats.load_px_checkpoint(checkpoint_file, dim)
+        if ck is not None:
+            focused, start = ck
+    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
+    step = dim - start if not checkpoint_every else int(checkpoint_every)
+    i = start
+    while i < dim:
+        j = min(dim, i + max(step, 1))
+        focused[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision)
+        i = j
+        if checkpoint_file and checkpoint_every and i < dim:
+            formats.save_px_checkpoint(checkpoint_file, focused, i)
+    if checkpoint_file and os.path.exists(checkpoint_file):
+        try:
+            os.remove(checkpoint_file)
+        except OSError:
+            pass
+    return np.reshape(focused, (dimx, dimy), order="F")

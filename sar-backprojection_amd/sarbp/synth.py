@@ -1,0 +1,233 @@
+"""Seeded synthetic scenes in the reference's data model.
+
+A scene is the 13 inputs that ``FocalizzatoreBpPx`` takes (sar_focuser_puntual.py:160): DEM
+``x, y, z`` (float64, shape (DimRg, DimAz)), ``mask`` (uint8), ``index``/``sint`` (int32), the
+trajectory ``tx, ty, tz`` (float64[Naz]), RawData (complex64, shape (Nra, Naz), column-major as
+the reference stores it, sar_focuser_puntual.py:480-491) and the scalars near range, range
+spacing and wavelength.
+
+Workloads:
+
+* :func:`small_scene`  -- 500 x 50 pixels over a 5001-pulse track, 600 range bins, 40 targets.
+* :func:`full_scene`   -- 4000 x 250 pixels, 40001 pulses at 0.1 m, 4762 range bins,
+  2000 targets; ``strip`` keeps only the pixel columns of a sub-range for CPU checks.
+* :func:`tiny_scene`   -- a few-hundred-pixel scene for unit tests.
+
+Raw echoes are ``A * sinc((r - R)/dra) * exp(-j*4*pi*R/lambda)`` per target (r = near + bin*dra),
+truncated to +-``SINC_HALF`` bins around the peak and to pulses within a +-3 degree squint beam,
+plus circular complex Gaussian noise at the stated SNR (noise power = mean signal power / SNR).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+SINC_HALF = 8  # range bins either side of a target's peak that receive its echo
+BEAM_DEG = 3.0  # squint half-width of the illuminating beam (degrees)
+
+
+@dataclasses.dataclass
+class Scene:
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    mask: np.ndarray
+    index: np.ndarray
+    sint: np.ndarray
+    tx: np.ndarray
+    ty: np.ndarray
+    tz: np.ndarray
+    raw: np.ndarray  # complex64 (Nra, Naz), Fortran-ordered
+    near: float
+    dra: float
+    lam: float
+    targets: np.ndarray | None = None  # (ntarget, 4): x, y, z, amplitude
+    name: str = ""
+
+    @property
+    def shape(self):
+        return self.x.shape
+
+    @property
+    def Nra(self) -> int:
+        return int(self.raw.shape[0])
+
+    @property
+    def Naz(self) -> int:
+        return int(self.raw.shape[1])
+
+    def args(self):
+        """Arguments in FocalizzatoreBpPx order (sar_focuser_puntual.py:160)."""
+        return (self.x, self.y, self.z, self.mask, self.index, self.tx, self.ty, self.tz, self.raw,
+                self.near, self.dra, self.lam, self.sint)
+
+    def flat(self):
+        """F-order flattened pixel arrays (x, y, z, mask, index, sint)."""
+        f = lambda a: np.ascontiguousarray(np.asarray(a).flatten(order="F"))
+        return f(self.x), f(self.y), f(self.z), f(self.mask), f(self.index), f(self.sint)
+
+    def sub(self, ix0: int, ix1: int, iy0: int = 0, iy1: int | None = None) -> "Scene":
+        """The pixel sub-grid [ix0:ix1, iy0:iy1] sharing this scene's raw data and trajectory."""
+        sl = (slice(ix0, ix1), slice(iy0, iy1))
+        g = lambda a: np.asfortranarray(a[sl])
+        return dataclasses.replace(self, x=g(self.x), y=g(self.y), z=g(self.z), mask=g(self.mask),
+                                   index=g(self.index), sint=g(self.sint),
+                                   name=f"{self.name}[{ix0}:{ix1},{iy0}:{iy1}]")
+
+
+def _simulate_raw(rng, targets, tx, ty, tz, Nra, near, dra, lam, snr_db):
+    Naz = tx.size
+    raw = np.zeros((Nra, Naz), dtype=np.complex64, order="F")
+    offs = np.arange(-SINC_HALF, SINC_HALF + 1)
+    tanb = math.tan(math.radians(BEAM_DEG))
+    for (px, py, pz, amp) in targets:
+        # pulses whose along-track offset is inside the beam at this target's ground range
+        ground = math.hypot(py - float(np.mean(ty)), pz - float(np.mean(tz)))
+        half = tanb * ground
+        sel = np.nonzero(np.abs(tx - px) <= half)[0]
+        if sel.size == 0:
+            continue
+        R = np.sqrt((px - tx[sel]) ** 2 + (py - ty[sel]) ** 2 + (pz - tz[sel]) ** 2)
+        b0 = np.rint((R - near) / dra).astype(np.int64)
+        rb = b0[:, None] + offs[None, :]
+        ok = (rb >= 0) & (rb < Nra)
+        rr = near + rb * dra
+        val = amp * np.sinc((rr - R[:, None]) / dra) * np.exp(-1j * 4.0 * math.pi * R / lam)[:, None]
+        pa = np.broadcast_to(sel[:, None], rb.shape)
+        raw[rb[ok], pa[ok]] += val[ok].astype(np.complex64)
+    p_sig = float(np.mean(np.abs(raw.astype(np.complex128, copy=False)) ** 2)) if raw.size < 5e7 else \
+        float(np.mean(np.abs(raw[:, :: max(1, Naz // 2000)]) ** 2))
+    if snr_db is not None and p_sig > 0:
+        sigma = math.sqrt(p_sig / (10.0 ** (snr_db / 10.0)) / 2.0)
+        for a0 in range(0, Naz, 4096):  # chunked to bound memory
+            a1 = min(Naz, a0 + 4096)
+            noise = rng.standard_normal((2, Nra, a1 - a0), dtype=np.float32) * np.float32(sigma)
+            raw[:, a0:a1] += noise[0] + 1j * noise[1]
+    return raw
+
+
+def _mask(rng, shape, p=0.95):
+    return np.asfortranarray((rng.random(shape) < p).astype(np.uint8))
+
+
+def small_scene(seed: int = 0, snr_db: float | None = 20.0) -> Scene:
+    """500 x 50 pixels, 5001 pulses, 600 range bins, 40 targets, Sint = 400.
+
+    DEM x = 0..2495 m and y = 0..245 m at 5 m, z = 20*exp(-((x-1250)^2+(y-125)^2)/(2*300^2)).
+    Trajectory x = linspace(-500, 3000, 5001), y = -3000, z = 3000, each with N(0, 0.05^2) jitter.
+    lambda = 0.031 m, near = 4000 m, dra = 0.5 m.  Index = the first pulse whose squint from the
+    pixel is within +-3 degrees; mask ~ Bernoulli(0.95)."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(500) * 5.0
+    ys = np.arange(50) * 5.0
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    Z = 20.0 * np.exp(-((X - 1250.0) ** 2 + (Y - 125.0) ** 2) / (2.0 * 300.0 ** 2))
+    Naz = 5001
+    tx = np.linspace(-500.0, 3000.0, Naz) + rng.normal(0, 0.05, Naz)
+    ty = -3000.0 + rng.normal(0, 0.05, Naz)
+    tz = 3000.0 + rng.normal(0, 0.05, Naz)
+    near, dra, lam, Nra = 4000.0, 0.5, 0.031, 600
+    # Index: first pulse with |squint| <= 3 deg.  squint = asin(along-track offset / slant range)
+    sin3 = math.sin(math.radians(3.0))
+    index = np.zeros(X.shape, np.int32)
+    for j in range(X.shape[1]):
+        dxs = tx[None, :] - X[:, j:j + 1]
+        R = np.sqrt(dxs ** 2 + (ty[None, :] - Y[:, j:j + 1]) ** 2 + (tz[None, :] - Z[:, j:j + 1]) ** 2)
+        inside = np.abs(dxs / R) <= sin3
+        index[:, j] = np.argmax(inside, axis=1)
+    ntg = 40
+    tgx = rng.uniform(0, xs[-1], ntg)
+    tgy = rng.uniform(0, ys[-1], ntg)
+    tgz = 20.0 * np.exp(-((tgx - 1250.0) ** 2 + (tgy - 125.0) ** 2) / (2.0 * 300.0 ** 2))
+    amp = np.abs(rng.normal(0, 1, ntg))
+    targets = np.stack([tgx, tgy, tgz, amp], axis=1)
+    raw = _simulate_raw(rng, targets, tx, ty, tz, Nra, near, dra, lam, snr_db)
+    sint = np.full(X.shape, 400, np.int32, order="F")
+    return Scene(np.asfortranarray(X), np.asfortranarray(Y), np.asfortranarray(Z),
+                 _mask(rng, X.shape), np.asfortranarray(index), sint, tx, ty, tz, raw,
+                 near, dra, lam, targets, "small-500x50")
+
+
+def hills(X, Y, rng, nhill=8, zmax=50.0):
+    """Smooth random DEM: sum of ``nhill`` Gaussian hills scaled to [0, zmax] m."""
+    Z = np.zeros_like(X)
+    x0, x1 = float(X.min()), float(X.max())
+    y0, y1 = float(Y.min()), float(Y.max())
+    for _ in range(nhill):
+        cx, cy = rng.uniform(x0, x1), rng.uniform(y0, y1)
+        s = rng.uniform(150.0, 600.0)
+        a = rng.uniform(0.3, 1.0)
+        Z += a * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * s * s))
+    return Z * (zmax / max(float(Z.max()), 1e-12))
+
+
+def full_scene(seed: int = 1, snr_db: float | None = 20.0, ntarget: int = 2000,
+               nx: int = 4000, ny: int = 250, naz: int = 40001, nra: int = 4762) -> Scene:
+    """4000 x 250 pixels at 1 m (x) x 4 m (y) on a DEM of 8 Gaussian hills (0-50 m).
+
+    40001 pulses along x at 0.1 m spacing, altitude 3000 m, cross-track offset -3000 m.
+    RawData 4762 x 40001 complex64 (~1.5 GB), 2000 targets + 20 dB noise; lambda = 0.031 m,
+    dra = 0.5 m, near = 4000 m.  Index = the broadside pulse (round(x/0.1)); Sint ~ U[2500,3500];
+    mask ~ Bernoulli(0.95).  ``nx/ny/naz/nra`` scale the same geometry down for tests."""
+    rng = np.random.default_rng(seed)
+    xs = np.arange(nx) * 1.0
+    ys = np.arange(ny) * 4.0
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    Z = hills(X, Y, rng)
+    tx = np.arange(naz) * 0.1
+    ty = np.full(naz, -3000.0)
+    tz = np.full(naz, 3000.0)
+    near, dra, lam = 4000.0, 0.5, 0.031
+    index = np.asfortranarray(np.clip(np.rint(X / 0.1), 0, naz - 1).astype(np.int32))
+    sint = np.asfortranarray(rng.integers(2500, 3501, X.shape).astype(np.int32))
+    tgx = rng.uniform(0, xs[-1], ntarget)
+    tgy = rng.uniform(0, ys[-1], ntarget)
+    # target heights from the same DEM (nearest pixel)
+    tgz = Z[np.clip(np.rint(tgx).astype(int), 0, nx - 1), np.clip(np.rint(tgy / 4).astype(int), 0, ny - 1)]
+    amp = np.abs(rng.normal(0, 1, ntarget))
+    targets = np.stack([tgx, tgy, tgz, amp], axis=1)
+    raw = _simulate_raw(rng, targets, tx, ty, tz, nra, near, dra, lam, snr_db)
+    return Scene(np.asfortranarray(X), np.asfortranarray(Y), np.asfortranarray(Z), _mask(rng, X.shape),
+                 index, sint, tx, ty, tz, raw, near, dra, lam, targets, f"full-{nx}x{ny}x{naz}")
+
+
+def tiny_scene(seed: int = 2, nx: int = 24, ny: int = 12, naz: int = 801, nra: int = 80,
+               sint_lo: int = 40, sint_hi: int = 120, ntarget: int = 4, snr_db: float | None = 20.0,
+               edge_cases: bool = True) -> Scene:
+    """A small scene in the full-scene geometry scaled down (pixel 1 m x 4 m, pulses 0.1 m apart,
+    R ~ 4.2-4.3 km so the phase magnitude matches the full workload).  With ``edge_cases`` it
+    plants Sint = -9999 / -1 / -2 / 0 / 1, masked pixels, apertures clipped at both track ends
+    and pixels beyond the last range bin."""
+    rng = np.random.default_rng(seed)
+    xs = 30.0 + np.arange(nx) * 1.0
+    ys = np.arange(ny) * 4.0
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    Z = hills(X, Y, rng, nhill=3, zmax=20.0)
+    tx = np.arange(naz) * 0.1 + rng.normal(0, 0.02, naz)
+    ty = np.full(naz, -3000.0) + rng.normal(0, 0.02, naz)
+    tz = np.full(naz, 3000.0) + rng.normal(0, 0.02, naz)
+    near, dra, lam = 4230.0, 0.5, 0.031
+    index = np.asfortranarray(np.clip(np.rint(X / 0.1), 0, naz - 1).astype(np.int32))
+    sint = np.asfortranarray(rng.integers(sint_lo, sint_hi + 1, X.shape).astype(np.int32))
+    mask = _mask(rng, X.shape)
+    if edge_cases:
+        flat_s = sint.reshape(-1, order="F")
+        specials = [-9999, -1, -2, 0, 1, -3, 2, 5]
+        for k, s in enumerate(specials):
+            sint[k % nx, (k * 3) % ny] = s
+        mask[1, 1] = 0
+        index[0, 2] = 0           # aperture clipped at the track start
+        index[nx - 1, 3] = naz - 1  # ... and at the track end
+        index[2, 4] = -50         # centre before the first pulse
+        index[3, 5] = naz + 30    # centre after the last pulse
+        del flat_s
+    tgx = rng.uniform(xs[0], xs[-1], ntarget)
+    tgy = rng.uniform(ys[0], ys[-1], ntarget)
+    tgz = Z[np.clip(np.rint(tgx - xs[0]).astype(int), 0, nx - 1), np.clip(np.rint(tgy / 4).astype(int), 0, ny - 1)]
+    amp = np.abs(rng.normal(0, 1, ntarget)) + 0.5
+    targets = np.stack([tgx, tgy, tgz, amp], axis=1)
+    raw = _simulate_raw(rng, targets, tx, ty, tz, nra, near, dra, lam, snr_db)
+    return Scene(np.asfortranarray(X), np.asfortranarray(Y), np.asfortranarray(Z), mask, index, sint,
+                 tx, ty, tz, raw, near, dra, lam, targets, f"tiny-{nx}x{ny}")

@@ -1,0 +1,63 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares (no GPU needed)."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from sarbp import _lib
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"\b(sarbp_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    # typedef'd callback type names are not functions
+    return {n for n in names if not n.endswith("_fn")}
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert names, "no declarations found"
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declaration():
+    assert declared_functions() == set(_lib.SIGNATURES)
+
+
+def test_abi_version_and_strerror():
+    L = _lib.load()
+    assert L.sarbp_abi_version() == 1
+    assert L.sarbp_strerror(0) == b"ok"
+    assert L.sarbp_strerror(-1) == b"invalid argument"
+
+
+def test_host_pp_count_matches_oracle():
+    from oracle import oracle_lib as O
+    rng = np.random.default_rng(4)
+    n, naz = 2000, 500
+    idx = rng.integers(-60, naz + 60, n).astype(np.int32)
+    s = rng.integers(-8, 300, n).astype(np.int32)
+    s[::13] = -9999
+    m = (rng.random(n) < 0.9).astype(np.uint8)
+    from sarbp.core import count_pp
+    assert count_pp(m, idx, s, naz) == O.count_pp(m, idx, s, naz)
+
+
+def test_null_arguments_rejected_without_device_work():
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    st = L.sarbp_dataset_create(None, 10, 10, None, None, None, 4000.0, 0.5, 0.031, ctypes.byref(h))
+    assert st == _lib.EINVAL and not h.value
+    assert b"null" in L.sarbp_last_error()
+    assert L.sarbp_scene_create(None, None, None, None, None, None, None, 0, ctypes.byref(h)) == _lib.EINVAL
+    assert L.sarbp_dataset_destroy(None) == 0

@@ -1,0 +1,131 @@
+"""FP64 HIP path vs the CPU oracle, through the C ABI (GPU).
+
+Bar: per-pixel relative error <= 1e-10, contributor counts bit-exact, pixels without
+contributors exactly 0 on both sides.  A single range bin chosen differently in a pixel changes it
+by ~1/sqrt(n) (percent level), so passing this bound also shows the bin selection is exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle_lib as O
+from sarbp import synth
+from sarbp.core import Dataset
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-10
+
+
+def check_parity(got, cnt, ref, rcnt, tol=REL_TOL):
+    got, ref = np.ravel(got), np.ravel(ref)
+    cnt, rcnt = np.ravel(cnt), np.ravel(rcnt)
+    assert np.array_equal(cnt, rcnt), f"contributor counts differ at {np.nonzero(cnt != rcnt)[0][:10]}"
+    zero = rcnt == 0
+    assert np.all(got[zero] == 0) and np.all(ref[zero] == 0)
+    nz = ~zero
+    if nz.any():
+        rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
+        assert rel.max() <= tol, f"max rel err {rel.max():.3e} at pixel {np.nonzero(nz)[0][np.argmax(rel)]}"
+        return float(rel.max())
+    return 0.0
+
+
+def focus_both(sc, precision="f64"):
+    ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    try:
+        xf, yf, zf, mf, inf, sf = sc.flat()
+        got, cnt = ds.focus_flat(xf, yf, zf, mf, inf, sf, precision=precision, with_counts=True)
+    finally:
+        ds.close()
+    ref, rcnt = O.focus_flat(*sc.flat(), sc.tx, sc.ty, sc.tz, sc.raw, sc.near, sc.dra, sc.lam)
+    return got, cnt, ref, rcnt
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_tiny_scene_with_edge_cases(gpu, seed):
+    sc = synth.tiny_scene(seed=seed)
+    print(f"case tiny seed={seed} shape={sc.shape}", flush=True)
+    got, cnt, ref, rcnt = focus_both(sc)
+    check_parity(got, cnt, ref, rcnt)
+    assert (rcnt == 0).any() and (rcnt > 0).any()
+
+
+def test_range_bin_ties_on_gpu(gpu):
+    from test_oracle import TIES, _tie_setup
+    heights = [h for h, _ in TIES]
+    tx, ty, tz, raw = _tie_setup(heights)
+    n = len(heights)
+    ds = Dataset(raw, tx, ty, tz, 4000.0, 0.5, 0.031)
+    got, cnt = ds.focus_flat(np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n, np.uint8),
+                             np.arange(1, n + 1, dtype=np.int32), np.zeros(n, np.int32), with_counts=True)
+    ds.close()
+    for k, (h, want) in enumerate(TIES):
+        if want is None:
+            assert cnt[k] == 0 and got[k] == 0, h
+        else:
+            assert cnt[k] == 1, h
+            assert round(abs(got[k]) / (h * h)) - 1 == want, h
+
+
+def test_small_scene_full(gpu):
+    sc = synth.small_scene(seed=0)
+    print(f"case small shape={sc.shape} raw={sc.raw.shape}", flush=True)
+    got, cnt, ref, rcnt = focus_both(sc)
+    check_parity(got, cnt, ref, rcnt)
+
+
+def test_full_geometry_strip(gpu):
+    # the full workload's geometry and magnitudes (R 4.2-5 km, Sint 2500-3500, 4762 range bins)
+    # over a 5121-pulse track; a 64 x 250 strip in the middle is checked against the oracle
+    sc = synth.full_scene(seed=4, nx=512, ny=250, naz=5121, nra=4762, ntarget=150)
+    strip = sc.sub(224, 288)
+    print(f"case full-geometry strip shape={strip.shape}", flush=True)
+    got, cnt, ref, rcnt = focus_both(strip)
+    check_parity(got, cnt, ref, rcnt)
+
+
+def test_dropin_focalizzatore_and_process_point(gpu, tmp_path):
+    from sarbp.sar_focuser_puntual import FocalizzatoreBpPx, process_point
+    sc = synth.tiny_scene(seed=6)
+    ref, rcnt = O.focus(*sc.args())
+    os.chdir(tmp_path)
+    got = FocalizzatoreBpPx(*sc.args())
+    assert got.shape == sc.shape and got.dtype == np.complex128
+    check_parity(got, rcnt, ref, rcnt)
+    # resumable pass with intermediate checkpoints gives the same image and removes the file
+    got2 = FocalizzatoreBpPx(*sc.args(), checkpoint_every=37)
+    assert np.array_equal(got2, got)
+    assert not os.path.exists("focalizador_px_checkpoint.npz")
+    xf, yf, zf, mf, inf, sf = sc.flat()
+    for i in (0, 7, 100, xf.size - 1):
+        v = process_point(xf[i], yf[i], zf[i], mf[i], inf[i], sf[i], sc.tx, sc.ty, sc.tz, sc.raw, sc.near, sc.dra,
+                          sc.lam, sc.Nra, sc.Naz)
+        r = ref.flatten(order="F")[i]
+        assert (v == 0 and r == 0) or abs(v - r) <= REL_TOL * abs(r)
+    with pytest.raises(ValueError):
+        FocalizzatoreBpPx(sc.x, sc.y, sc.z, sc.mask, sc.index, sc.tx, sc.ty, sc.tz, sc.raw.ravel(), sc.near,
+                          sc.dra, sc.lam, sc.sint)
+    with pytest.raises(ValueError):
+        FocalizzatoreBpPx(sc.x.ravel(), sc.y, sc.z, sc.mask, sc.index, sc.tx, sc.ty, sc.tz, sc.raw, sc.near,
+                          sc.dra, sc.lam, sc.sint)
+
+
+def test_resume_from_existing_checkpoint(gpu, tmp_path):
+    from sarbp import formats
+    from sarbp.sar_focuser_puntual import FocalizzatoreBpPx
+    sc = synth.tiny_scene(seed=8)
+    os.chdir(tmp_path)
+    full = FocalizzatoreBpPx(*sc.args())
+    flat = full.flatten(order="F")
+    k = flat.size // 3
+    saved = flat.copy()
+    saved[k:] = 0
+    formats.save_px_checkpoint("focalizador_px_checkpoint.npz", saved, k)
+    again = FocalizzatoreBpPx(*sc.args())
+    assert np.array_equal(again, full)
+    assert not os.path.exists("focalizador_px_checkpoint.npz")
+    # a checkpoint of another size is ignored (fresh start)
+    formats.save_px_checkpoint("focalizador_px_checkpoint.npz", np.ones(5, np.complex128), 3)
+    assert np.array_equal(FocalizzatoreBpPx(*sc.args()), full)
