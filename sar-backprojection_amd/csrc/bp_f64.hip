@@ -6,13 +6,52 @@
 // order; this file is compiled with -ffp-contract=off so no multiply-add is fused, and sqrt and
 // division are the correctly rounded f64 expansions.  The only difference left from the CPU path
 // is the last-ulp behaviour of the device sin/cos.
+//
+// Two kernels:
+//   k_bp_f64_lane  one pixel per lane, each lane walking its own aperture (no tiling);
+//   k_bp_f64_tile  one 64-pixel tile per wavefront, walking the union of the tile's apertures
+//                  pulse by pulse: the pulse index is wave-uniform, so the trajectory sample is a
+//                  scalar load and the RawData gathers of neighbouring pixels fall in the same or
+//                  adjacent range bins of one pulse column (few cache lines per load).
+// Both accumulate each pixel's terms in ascending pulse order, as the reference does.
 #include "sarbp_internal.h"
 
 #pragma clang fp contract(off)
 
 namespace {
 
-// One pixel per lane, each lane walking its own aperture in ascending pulse order.
+// One pixel-pulse term of process_point (sar_focuser_puntual.py:82-146).  Returns false when the
+// range bin falls outside [0, nra-1).
+__device__ __forceinline__ bool f64_term(const BpGeom &g, double x, double y, double z, double tx, double ty,
+                                         double tz, const float2 *__restrict__ col, double &acc_re,
+                                         double &acc_im) {
+  const double dx = x - tx;                       // :82-84
+  const double dy = y - ty;
+  const double dz = z - tz;
+  const double r2 = dx * dx + dy * dy + dz * dz;  // :88
+  const double r = __builtin_sqrt(r2);            // :94
+  const double q = __builtin_rint((r - g.near_range) / g.dra);  // :100-101, half to even
+  if (!(q < (double)(g.nra - 1) && q >= 0.0)) return false;      // :106
+  const double ph = g.kphase * r + g.beta * r2;   // :116
+  double sn, cs;
+  sincos(ph, &sn, &cs);                           // exp(-j ph) = (cs, -sn), :121
+  const float2 d = col[(int)q];
+  const double t1r = r2 * (double)d.x;            // :144, R2 * raw
+  const double t1i = r2 * (double)d.y;
+  acc_re += t1r * cs + t1i * sn;                  // t1r*cs - t1i*(-sn)
+  acc_im += t1i * cs - t1r * sn;                  // t1r*(-sn) + t1i*cs
+  return true;
+}
+
+__device__ __forceinline__ void store_mean(double2 *out, int32_t *ncnt, int64_t i, double acc_re, double acc_im,
+                                           int32_t cnt) {
+  double2 o;
+  o.x = cnt > 0 ? acc_re / (double)cnt : 0.0;  // :150-152, componentwise mean
+  o.y = cnt > 0 ? acc_im / (double)cnt : 0.0;
+  out[i] = o;
+  ncnt[i] = cnt;
+}
+
 __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, double2 *__restrict__ out,
                                                      int32_t *__restrict__ ncnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -20,47 +59,74 @@ __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, do
   double acc_re = 0.0, acc_im = 0.0;
   int32_t cnt = 0;
   const int32_t s = px.sint[i];
-  if (px.mask[i] == 1 && s != -9999) {          // sar_focuser_puntual.py:46-53, 208-211
+  if (px.mask[i] == 1 && s != -9999) {  // sar_focuser_puntual.py:46-53, 208-211
     int64_t lo, hi;
     sarbp_pulse_range(px.index[i], s, g.naz, lo, hi);
     const double x = px.x[i], y = px.y[i], z = px.z[i];
-    const double lim = (double)(g.nra - 1);
     for (int64_t a = lo; a < hi; ++a) {
       const double4 t = g.traj[a];
-      const double dx = x - t.x;                 // sar_focuser_puntual.py:82-84
-      const double dy = y - t.y;
-      const double dz = z - t.z;
-      const double r2 = dx * dx + dy * dy + dz * dz;  // :88
-      const double r = __builtin_sqrt(r2);            // :94
-      const double q = __builtin_rint((r - g.near_range) / g.dra);  // :100-101, half to even
-      if (!(q < lim && q >= 0.0)) continue;           // :106
-      const double ph = g.kphase * r + g.beta * r2;   // :116
-      double sn, cs;
-      sincos(ph, &sn, &cs);                           // exp(-j ph) = (cs, -sn), :121
-      const float2 d = g.raw[(int64_t)q + g.nra * a];
-      const double t1r = r2 * (double)d.x;            // :144, R2 * raw
-      const double t1i = r2 * (double)d.y;
-      acc_re += t1r * cs + t1i * sn;                  // t1r*cs - t1i*(-sn)
-      acc_im += t1i * cs - t1r * sn;                  // t1r*(-sn) + t1i*cs
-      ++cnt;
+      if (f64_term(g, x, y, z, t.x, t.y, t.z, g.raw + g.nra * a, acc_re, acc_im)) ++cnt;
     }
   }
-  double2 o;
-  o.x = cnt > 0 ? acc_re / (double)cnt : 0.0;        // :150-152, componentwise mean
-  o.y = cnt > 0 ? acc_im / (double)cnt : 0.0;
-  out[i] = o;
-  ncnt[i] = cnt;
+  store_mean(out, ncnt, i, acc_re, acc_im, cnt);
+}
+
+// One wavefront per tile; block = 64 threads.
+__global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, TileGrid tg, double2 *__restrict__ out,
+                                                    int32_t *__restrict__ ncnt) {
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= tg.ntiles()) return;  // whole wavefront
+  const int lane = threadIdx.x;
+  const int64_t ix = (tile % tg.ntx) * tg.tx + lane % tg.tx;
+  const int64_t iy = (tile / tg.ntx) * tg.ty + lane / tg.tx;
+  const bool valid = ix < tg.dimx && iy < tg.dimy;
+  const int64_t i = ix + tg.dimx * iy;
+  double x = 0, y = 0, z = 0;
+  int lo = 0x7fffffff, hi = -0x7fffffff;  // empty aperture
+  if (valid) {
+    const int32_t s = px.sint[i];
+    if (px.mask[i] == 1 && s != -9999) {  // sar_focuser_puntual.py:46-53
+      int64_t l, h;
+      sarbp_pulse_range(px.index[i], s, g.naz, l, h);
+      if (h > l) {
+        lo = (int)l;
+        hi = (int)h;
+      }
+      x = px.x[i];
+      y = px.y[i];
+      z = px.z[i];
+    }
+  }
+  const int wlo = __builtin_amdgcn_readfirstlane(wave_min_i32(lo));
+  const int whi = __builtin_amdgcn_readfirstlane(wave_max_i32(hi));
+  double acc_re = 0.0, acc_im = 0.0;
+  int32_t cnt = 0;
+  for (int a = wlo; a < whi; ++a) {  // wave-uniform pulse
+    const double4 t = g.traj[a];     // scalar load
+    if (a >= lo && a < hi) {
+      if (f64_term(g, x, y, z, t.x, t.y, t.z, g.raw + g.nra * (int64_t)a, acc_re, acc_im)) ++cnt;
+    }
+  }
+  if (valid) store_mean(out, ncnt, i, acc_re, acc_im, cnt);
 }
 
 }  // namespace
 
-int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, hipStream_t st) {
+int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st) {
   if (sc->n == 0) return SARBP_OK;
-  const int64_t blocks = (sc->n + 255) / 256;
-  if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many pixels for one launch");
-  hipLaunchKernelGGL(k_bp_f64_lane, dim3((unsigned)blocks), dim3(256), 0, st, ds->geom(), sc->view(),
-                     sc->out, sc->ncnt);
+  if (t.kernel == 0) {
+    const int64_t blocks = (sc->n + 255) / 256;
+    if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many pixels for one launch");
+    hipLaunchKernelGGL(k_bp_f64_lane, dim3((unsigned)blocks), dim3(256), 0, st, ds->geom(), sc->view(), sc->out,
+                       sc->ncnt);
+  } else {
+    const TileGrid tg = make_tile_grid(sc->dimx, sc->dimy, t.tile_x);
+    const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
+    if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
+    hipLaunchKernelGGL(k_bp_f64_tile, dim3((unsigned)blocks), dim3(64), 0, st, ds->geom(), sc->view(), tg, sc->out,
+                       sc->ncnt);
+  }
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return sarbp_hip_fail(e, "k_bp_f64_lane launch");
+  if (e != hipSuccess) return sarbp_hip_fail(e, "FP64 kernel launch");
   return SARBP_OK;
 }

@@ -50,9 +50,21 @@ struct sarbp_dataset {
   BpGeom geom() const;
 };
 
+// How a focusing pass maps pixels to wavefronts.  A tile is tile_x * (64/tile_x) pixels of the
+// F-order grid (dimx fast); one 64-lane wavefront walks the union of its pixels' apertures pulse by
+// pulse.  kernel 0 = one pixel per lane with per-lane apertures (no tiling).
+struct Tiling {
+  int kernel = 1;       // 0: per-lane apertures, 1: pulse-synchronous tiles
+  int tile_x = 16;      // 64, 32, 16 or 8
+  int chunk = 64;       // pulses of per-pulse reference data staged per LDS refill (split path)
+  int accum = 0;        // split path: 0 = Kahan FP32, 1 = FP32 blocks folded into FP64
+};
+
 struct sarbp_scene {
   sarbp_dataset *ds = nullptr;
   int64_t n = 0;
+  int64_t dimx = 0, dimy = 0;   // grid shape (F order); dimx = n, dimy = 1 for a plain list
+  Tiling tiling_f64, tiling_split;
   double *x = nullptr, *y = nullptr, *z = nullptr;
   uint8_t *mask = nullptr;
   int32_t *index = nullptr, *sint = nullptr;
@@ -87,4 +99,54 @@ int sarbp_fail(int code, const std::string &msg);
 int sarbp_hip_fail(hipError_t e, const char *what);
 
 // kernel launchers (bp_f64.hip, bp_split.hip)
-int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, hipStream_t st);
+int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st);
+int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st);
+
+// Grid tile -> lanes.  Tiles are numbered x-fastest inside a band of tile rows; blocks are remapped
+// so that each XCD (blocks are dealt round-robin to the 8 XCDs) works on a contiguous run of tiles
+// and its L2 sees neighbouring apertures.
+struct TileGrid {
+  int64_t dimx, dimy;   // pixel grid
+  int64_t ntx, nty;     // tiles along x and y
+  int tx, ty;           // tile shape, tx * ty == 64
+  int64_t ntiles() const { return ntx * nty; }
+};
+inline TileGrid make_tile_grid(int64_t dimx, int64_t dimy, int tile_x) {
+  TileGrid g;
+  g.dimx = dimx;
+  g.dimy = dimy;
+  g.tx = tile_x;
+  g.ty = 64 / tile_x;
+  g.ntx = (dimx + g.tx - 1) / g.tx;
+  g.nty = (dimy + g.ty - 1) / g.ty;
+  return g;
+}
+constexpr int SARBP_NUM_XCD = 8;
+__device__ inline int64_t xcd_tile(int64_t block, int64_t nblocks) {
+  // block b runs on XCD b % 8; give XCD k the tiles [k*per, (k+1)*per)
+  const int64_t per = (nblocks + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD;
+  const int64_t xcd = block % SARBP_NUM_XCD, slot = block / SARBP_NUM_XCD;
+  return xcd * per + slot;
+}
+
+// wavefront reductions (64 lanes)
+__device__ inline int wave_min_i32(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = min(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ inline int wave_max_i32(int v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = max(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ inline double wave_min_f64(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ inline double wave_max_f64(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
