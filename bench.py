@@ -91,6 +91,9 @@ def main():
     ap.add_argument("--strip", type=int, default=256, help="pixel columns of the CPU baseline strip")
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
+    ap.add_argument("--f64-map", default="1,16", help="FP64 work mapping kernel,tile_x")
+    ap.add_argument("--split-map", default="16,0", help="split work mapping tile_x,accum")
+    ap.add_argument("--sweep", action="store_true", help="time every work mapping first and use the fastest")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pixel sharding across GPUs "
@@ -108,9 +111,28 @@ def main():
     log(f"scene {sc.name} generated in {time.perf_counter() - t0:.1f} s")
     ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
     xf, yf, zf, mf, inf, sf = sc.flat()
-    scene = ds.scene(xf, yf, zf, mf, inf, sf)
+    scene = ds.scene(xf, yf, zf, mf, inf, sf, dimx=sc.dimx)
     pp = scene.count_pp()
     log(f"dataset resident ({ds.device_bytes / 1e9:.2f} GB), {scene.n} pixels, {pp} pp")
+    k, tx = (int(v) for v in args.f64_map.split(","))
+    scene.set_tiling("f64", k, tx, 0)
+    tx, acc = (int(v) for v in args.split_map.split(","))
+    scene.set_tiling("split", 1, tx, acc)
+    sweep = {}
+    if args.sweep:
+        best = {}
+        for prec, maps in (("f64", [(0, 16, 0)] + [(1, t, 0) for t in (64, 32, 16, 8, 4)]),
+                           ("split", [(1, t, a) for t in (64, 32, 16, 8) for a in (0, 1)])):
+            for m in maps:
+                scene.set_tiling(prec, *m)
+                _, ks = time_precision(scene, prec, 2, 1)
+                sweep[f"{prec}:{m[0]},{m[1]},{m[2]}"] = pp / ks
+                log(f"sweep {prec} kernel={m[0]} tile_x={m[1]} accum={m[2]}: {pp / ks:.4e} pp/s (kernel time)")
+                if prec not in best or ks < best[prec][0]:
+                    best[prec] = (ks, m)
+        for prec, (_, m) in best.items():
+            scene.set_tiling(prec, *m)
+            log(f"using {prec} mapping {m}")
 
     res = {}
     outs = {}
@@ -125,7 +147,9 @@ def main():
         log(f"{prec}: {wall * 1e3:.2f} ms/step (kernel {gpu_s * 1e3:.2f} ms), {pp / wall:.4e} pp/s")
 
     head = "split" if "split" in res else "f64"
-    extra = {"pp_per_step": pp, "pixels": int(scene.n)}
+    extra = {"pp_per_step": pp, "pixels": int(scene.n), "f64_map": args.f64_map, "split_map": args.split_map}
+    if sweep:
+        extra["sweep_pp_per_s"] = sweep
     for p, r in res.items():
         extra[f"{p}_pp_per_s"] = r["pp_per_s"]
         extra[f"{p}_kernel_ms"] = r["kernel_s"] * 1e3

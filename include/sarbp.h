@@ -77,11 +77,20 @@ int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes);
 
 /*
  * Upload a pixel set (F-order flattened DEM grid) for repeated focusing.  Pixels whose sint is
- * -9999 are masked, as FocalizzatoreBpPx does (sar_focuser_puntual.py:208-211).
+ * -9999 are masked, as FocalizzatoreBpPx does (sar_focuser_puntual.py:208-211).  dimx is the
+ * grid's fast (first) dimension, so pixel (ix, iy) is element ix + dimx*iy; 0 treats the pixels
+ * as a plain list.  Tiles of neighbouring pixels are formed from this shape.
  */
 int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, const double *z,
                        const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
-                       sarbp_scene **out);
+                       int64_t dimx, sarbp_scene **out);
+/*
+ * Work mapping of one precision (tuning; the defaults need no call).  kernel 1 = one 64-pixel
+ * tile per wavefront walking the union of the tile's apertures pulse by pulse, tile_x x (64/tile_x)
+ * pixels; kernel 0 = one pixel per lane (FP64 only).  accum (split only): 0 = Kahan FP32,
+ * 1 = FP32 blocks of 64 pulses folded into FP64.
+ */
+int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_x, int accum);
 int sarbp_scene_destroy(sarbp_scene *sc);
 
 /*
@@ -104,7 +113,7 @@ int sarbp_scene_download(sarbp_scene *sc, double *out_c128, int32_t *ncontrib);
  */
 int sarbp_focus(sarbp_dataset *ds, const double *x, const double *y, const double *z,
                 const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
-                int precision, double *out_c128, int32_t *ncontrib);
+                int64_t dimx, int precision, double *out_c128, int32_t *ncontrib);
 
 /*
  * Pixel-pulse pairs evaluated by a focusing pass (host-side, integers only): pixels with

@@ -106,6 +106,14 @@ int sarbp_hip_fail(hipError_t e, const char *what) {
   return SARBP_EDEVICE;
 }
 
+Tiling default_tiling(int precision) {
+  Tiling t;
+  t.kernel = 1;
+  t.tile_x = 16;
+  t.accum = precision == SARBP_PREC_SPLIT ? 0 : 0;
+  return t;
+}
+
 BpGeom sarbp_dataset::geom() const {
   BpGeom g;
   g.raw = raw;
@@ -230,17 +238,23 @@ int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes) {
 }
 
 int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, const double *z, const uint8_t *mask,
-                       const int32_t *index, const int32_t *sint, int64_t n, sarbp_scene **out) {
+                       const int32_t *index, const int32_t *sint, int64_t n, int64_t dimx, sarbp_scene **out) {
   if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
   *out = nullptr;
   if (!ds) return sarbp_fail(SARBP_EINVAL, "dataset is null");
   if (n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
+  if (dimx < 0 || (n > 0 && dimx > 0 && n % dimx != 0))
+    return sarbp_fail(SARBP_EINVAL, "pixel count must be a multiple of the grid's fast dimension dimx");
   if (n > 0 && (!x || !y || !z || !mask || !index || !sint))
     return sarbp_fail(SARBP_EINVAL, "pixel array pointer is null");
   sarbp_scene *sc = new (std::nothrow) sarbp_scene();
   if (!sc) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
   sc->ds = ds;
   sc->n = n;
+  sc->dimx = dimx > 0 ? dimx : (n > 0 ? n : 1);
+  sc->dimy = n > 0 ? n / sc->dimx : 0;
+  sc->tiling_f64 = default_tiling(SARBP_PREC_F64);
+  sc->tiling_split = default_tiling(SARBP_PREC_SPLIT);
   const size_t un = (size_t)n;
   int rc = dev_alloc(&sc->x, un, sc->bytes, "x");
   if (!rc) rc = dev_alloc(&sc->y, un, sc->bytes, "y");
@@ -286,8 +300,8 @@ int sarbp_scene_focus(sarbp_dataset *ds, sarbp_scene *sc, int precision, float *
     return sarbp_fail(SARBP_EINVAL, "unknown precision");
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
-  if (precision == SARBP_PREC_SPLIT) return sarbp_fail(SARBP_EINVAL, "split precision is not available yet");
-  int rc = sarbp_launch_f64(ds, sc, ds->stream);
+  int rc = precision == SARBP_PREC_SPLIT ? sarbp_launch_split(ds, sc, sc->tiling_split, ds->stream)
+                                         : sarbp_launch_f64(ds, sc, sc->tiling_f64, ds->stream);
   if (rc) return rc;
   e = hipEventRecord(ds->ev1, ds->stream);
   if (e == hipSuccess) e = hipEventSynchronize(ds->ev1);
@@ -314,11 +328,28 @@ int sarbp_scene_download(sarbp_scene *sc, double *out_c128, int32_t *ncontrib) {
   return SARBP_OK;
 }
 
+int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_x, int accum) {
+  if (!sc) return sarbp_fail(SARBP_EINVAL, "null handle");
+  if (precision != SARBP_PREC_F64 && precision != SARBP_PREC_SPLIT)
+    return sarbp_fail(SARBP_EINVAL, "unknown precision");
+  if (tile_x != 64 && tile_x != 32 && tile_x != 16 && tile_x != 8 && tile_x != 4)
+    return sarbp_fail(SARBP_EINVAL, "tile_x must be 64, 32, 16, 8 or 4");
+  if (kernel < 0 || kernel > 1 || accum < 0 || accum > 1)
+    return sarbp_fail(SARBP_EINVAL, "kernel and accum must be 0 or 1");
+  if (precision == SARBP_PREC_SPLIT && kernel == 0)
+    return sarbp_fail(SARBP_EINVAL, "the split path always uses tiles");
+  Tiling &t = precision == SARBP_PREC_SPLIT ? sc->tiling_split : sc->tiling_f64;
+  t.kernel = kernel;
+  t.tile_x = tile_x;
+  t.accum = accum;
+  return SARBP_OK;
+}
+
 int sarbp_focus(sarbp_dataset *ds, const double *x, const double *y, const double *z, const uint8_t *mask,
-                const int32_t *index, const int32_t *sint, int64_t n, int precision, double *out_c128,
-                int32_t *ncontrib) {
+                const int32_t *index, const int32_t *sint, int64_t n, int64_t dimx, int precision,
+                double *out_c128, int32_t *ncontrib) {
   sarbp_scene *sc = nullptr;
-  int rc = sarbp_scene_create(ds, x, y, z, mask, index, sint, n, &sc);
+  int rc = sarbp_scene_create(ds, x, y, z, mask, index, sint, n, dimx, &sc);
   if (rc) return rc;
   rc = sarbp_scene_focus(ds, sc, precision, nullptr);
   if (!rc) rc = sarbp_scene_download(sc, out_c128, ncontrib);

@@ -94,6 +94,8 @@ __host__ __device__ inline void sarbp_pulse_range(int32_t index, int32_t sint, i
   if (J <= 0 || hi < lo) hi = lo;
 }
 
+Tiling default_tiling(int precision);
+
 // error plumbing (api.hip)
 int sarbp_fail(int code, const std::string &msg);
 int sarbp_hip_fail(hipError_t e, const char *what);
@@ -109,9 +111,11 @@ struct TileGrid {
   int64_t dimx, dimy;   // pixel grid
   int64_t ntx, nty;     // tiles along x and y
   int tx, ty;           // tile shape, tx * ty == 64
-  int64_t ntiles() const { return ntx * nty; }
+  __host__ __device__ int64_t ntiles() const { return ntx * nty; }
 };
 inline TileGrid make_tile_grid(int64_t dimx, int64_t dimy, int tile_x) {
+  // widen the tile along x when the grid has fewer rows than the tile
+  while (tile_x < 64 && 64 / tile_x > dimy) tile_x *= 2;
   TileGrid g;
   g.dimx = dimx;
   g.dimy = dimy;

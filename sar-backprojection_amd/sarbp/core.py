@@ -117,11 +117,12 @@ class Dataset:
     def __exit__(self, *exc):
         self.close()
 
-    def scene(self, x, y, z, mask, index, sint) -> "Scene":
-        return Scene(self, x, y, z, mask, index, sint)
+    def scene(self, x, y, z, mask, index, sint, dimx=0) -> "Scene":
+        return Scene(self, x, y, z, mask, index, sint, dimx)
 
-    def focus_flat(self, x, y, z, mask, index, sint, precision="f64", with_counts=False):
-        """Focus F-order flattened pixels; returns complex128[n] (and int32 counts)."""
+    def focus_flat(self, x, y, z, mask, index, sint, precision="f64", with_counts=False, dimx=0):
+        """Focus F-order flattened pixels of a grid whose fast dimension is ``dimx`` (0: a plain
+        list); returns complex128[n] (and int32 counts)."""
         x, y, z = _c(x, np.float64).ravel(), _c(y, np.float64).ravel(), _c(z, np.float64).ravel()
         mask, index, sint = _c(mask, np.uint8).ravel(), _c(index, np.int32).ravel(), _c(sint, np.int32).ravel()
         n = x.size
@@ -130,7 +131,7 @@ class Dataset:
         out = np.zeros(n, np.complex128)
         cnt = np.zeros(n, np.int32)
         check(_lib.load().sarbp_focus(self._h, x.ctypes.data, y.ctypes.data, z.ctypes.data, mask.ctypes.data,
-                                      index.ctypes.data, sint.ctypes.data, n, precision_code(precision),
+                                      index.ctypes.data, sint.ctypes.data, n, int(dimx), precision_code(precision),
                                       out.ctypes.data, cnt.ctypes.data), "sarbp_focus")
         return (out, cnt) if with_counts else out
 
@@ -138,8 +139,9 @@ class Dataset:
 class Scene:
     """A device-resident pixel set (sarbp_scene_create) focused repeatedly against one dataset."""
 
-    def __init__(self, ds: Dataset, x, y, z, mask, index, sint):
+    def __init__(self, ds: Dataset, x, y, z, mask, index, sint, dimx=0):
         self.ds = ds
+        self.dimx = int(dimx)
         self.x, self.y, self.z = _c(x, np.float64).ravel(), _c(y, np.float64).ravel(), _c(z, np.float64).ravel()
         self.mask = _c(mask, np.uint8).ravel()
         self.index = _c(index, np.int32).ravel()
@@ -150,8 +152,13 @@ class Scene:
         h = ctypes.c_void_p()
         check(_lib.load().sarbp_scene_create(ds.handle, self.x.ctypes.data, self.y.ctypes.data, self.z.ctypes.data,
                                              self.mask.ctypes.data, self.index.ctypes.data, self.sint.ctypes.data,
-                                             self.n, ctypes.byref(h)), "sarbp_scene_create")
+                                             self.n, self.dimx, ctypes.byref(h)), "sarbp_scene_create")
         self._h = h
+
+    def set_tiling(self, precision="f64", kernel=1, tile_x=16, accum=0):
+        """Work mapping for one precision (sarbp_scene_set_tiling)."""
+        check(_lib.load().sarbp_scene_set_tiling(self._h, precision_code(precision), int(kernel), int(tile_x),
+                                                  int(accum)), "sarbp_scene_set_tiling")
 
     def count_pp(self) -> int:
         return count_pp(self.mask, self.index, self.sint, self.ds.naz)

@@ -91,11 +91,14 @@ def FocalizzatoreBpPx(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz, Raw_Data,
         if ck is not None:
             focused, start = ck
     ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
-    step = dim - start if not checkpoint_every else int(checkpoint_every)
+    # chunks of whole grid columns keep the tiles two-dimensional
+    step = dim - start if not checkpoint_every else -(-int(checkpoint_every) // dimx) * dimx
     i = start
     while i < dim:
         j = min(dim, i + max(step, 1))
-        focused[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision)
+        cols = dimx if (i % dimx == 0 and (j - i) % dimx == 0) else 0
+        focused[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision,
+                                     dimx=cols)
         i = j
         if checkpoint_file and checkpoint_every and i < dim:
             formats.save_px_checkpoint(checkpoint_file, focused, i)

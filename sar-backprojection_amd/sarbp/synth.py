@@ -63,6 +63,10 @@ class Scene:
         return (self.x, self.y, self.z, self.mask, self.index, self.tx, self.ty, self.tz, self.raw,
                 self.near, self.dra, self.lam, self.sint)
 
+    @property
+    def dimx(self) -> int:
+        return int(self.x.shape[0])
+
     def flat(self):
         """F-order flattened pixel arrays (x, y, z, mask, index, sint)."""
         f = lambda a: np.ascontiguousarray(np.asarray(a).flatten(order="F"))

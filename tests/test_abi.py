@@ -59,5 +59,5 @@ def test_null_arguments_rejected_without_device_work():
     st = L.sarbp_dataset_create(None, 10, 10, None, None, None, 4000.0, 0.5, 0.031, ctypes.byref(h))
     assert st == _lib.EINVAL and not h.value
     assert b"null" in L.sarbp_last_error()
-    assert L.sarbp_scene_create(None, None, None, None, None, None, None, 0, ctypes.byref(h)) == _lib.EINVAL
+    assert L.sarbp_scene_create(None, None, None, None, None, None, None, 0, 0, ctypes.byref(h)) == _lib.EINVAL
     assert L.sarbp_dataset_destroy(None) == 0

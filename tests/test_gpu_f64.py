@@ -32,24 +32,52 @@ def check_parity(got, cnt, ref, rcnt, tol=REL_TOL):
     return 0.0
 
 
-def focus_both(sc, precision="f64"):
+def focus_both(sc, precision="f64", tilings=None):
+    """GPU result(s) and the oracle's for one scene.  With ``tilings`` (a list of
+    (kernel, tile_x, accum)), returns a list of GPU results, one per work mapping."""
     ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
     try:
-        xf, yf, zf, mf, inf, sf = sc.flat()
-        got, cnt = ds.focus_flat(xf, yf, zf, mf, inf, sf, precision=precision, with_counts=True)
+        if tilings is None:
+            got, cnt = ds.focus_flat(*sc.flat(), precision=precision, with_counts=True, dimx=sc.dimx)
+        else:
+            scene = ds.scene(*sc.flat(), dimx=sc.dimx)
+            got = []
+            for (k, tx, acc) in tilings:
+                scene.set_tiling(precision, k, tx, acc)
+                scene.focus(precision)
+                got.append(scene.result(with_counts=True))
+            scene.close()
+            cnt = None
     finally:
         ds.close()
     ref, rcnt = O.focus_flat(*sc.flat(), sc.tx, sc.ty, sc.tz, sc.raw, sc.near, sc.dra, sc.lam)
     return got, cnt, ref, rcnt
 
 
+F64_MAPPINGS = [(0, 16, 0), (1, 64, 0), (1, 32, 0), (1, 16, 0), (1, 8, 0), (1, 4, 0)]
+
+
 @pytest.mark.parametrize("seed", [2, 3])
 def test_tiny_scene_with_edge_cases(gpu, seed):
     sc = synth.tiny_scene(seed=seed)
     print(f"case tiny seed={seed} shape={sc.shape}", flush=True)
+    outs, _, ref, rcnt = focus_both(sc, tilings=F64_MAPPINGS)
+    for (got, cnt), m in zip(outs, F64_MAPPINGS):
+        print("mapping", m, flush=True)
+        check_parity(got, cnt, ref, rcnt)
+    assert (rcnt == 0).any() and (rcnt > 0).any()
+
+
+def test_plain_pixel_list_and_odd_grid(gpu):
+    # a plain list (dimx = 0) and a grid whose sides are not multiples of the tile
+    sc = synth.tiny_scene(seed=9, nx=37, ny=5)
+    print(f"case odd grid shape={sc.shape}", flush=True)
     got, cnt, ref, rcnt = focus_both(sc)
     check_parity(got, cnt, ref, rcnt)
-    assert (rcnt == 0).any() and (rcnt > 0).any()
+    ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    got2, cnt2 = ds.focus_flat(*sc.flat(), with_counts=True, dimx=0)
+    ds.close()
+    check_parity(got2, cnt2, ref, rcnt)
 
 
 def test_range_bin_ties_on_gpu(gpu):
@@ -58,15 +86,17 @@ def test_range_bin_ties_on_gpu(gpu):
     tx, ty, tz, raw = _tie_setup(heights)
     n = len(heights)
     ds = Dataset(raw, tx, ty, tz, 4000.0, 0.5, 0.031)
-    got, cnt = ds.focus_flat(np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n, np.uint8),
-                             np.arange(1, n + 1, dtype=np.int32), np.zeros(n, np.int32), with_counts=True)
+    for prec in ("f64", "split"):
+        got, cnt = ds.focus_flat(np.zeros(n), np.zeros(n), np.zeros(n), np.ones(n, np.uint8),
+                                 np.arange(1, n + 1, dtype=np.int32), np.zeros(n, np.int32), precision=prec,
+                                 with_counts=True)
+        for k, (h, want) in enumerate(TIES):
+            if want is None:
+                assert cnt[k] == 0 and got[k] == 0, (prec, h)
+            else:
+                assert cnt[k] == 1, (prec, h)
+                assert round(abs(got[k]) / (h * h)) - 1 == want, (prec, h)
     ds.close()
-    for k, (h, want) in enumerate(TIES):
-        if want is None:
-            assert cnt[k] == 0 and got[k] == 0, h
-        else:
-            assert cnt[k] == 1, h
-            assert round(abs(got[k]) / (h * h)) - 1 == want, h
 
 
 def test_small_scene_full(gpu):
@@ -82,8 +112,10 @@ def test_full_geometry_strip(gpu):
     sc = synth.full_scene(seed=4, nx=512, ny=250, naz=5121, nra=4762, ntarget=150)
     strip = sc.sub(224, 288)
     print(f"case full-geometry strip shape={strip.shape}", flush=True)
-    got, cnt, ref, rcnt = focus_both(strip)
-    check_parity(got, cnt, ref, rcnt)
+    outs, _, ref, rcnt = focus_both(strip, tilings=F64_MAPPINGS)
+    for (got, cnt), m in zip(outs, F64_MAPPINGS):
+        print("mapping", m, flush=True)
+        check_parity(got, cnt, ref, rcnt)
 
 
 def test_dropin_focalizzatore_and_process_point(gpu, tmp_path):
