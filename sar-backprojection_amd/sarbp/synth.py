@@ -217,16 +217,14 @@ def tiny_scene(seed: int = 2, nx: int = 24, ny: int = 12, naz: int = 801, nra: i
     sint = np.asfortranarray(rng.integers(sint_lo, sint_hi + 1, X.shape).astype(np.int32))
     mask = _mask(rng, X.shape)
     if edge_cases:
-        flat_s = sint.reshape(-1, order="F")
         specials = [-9999, -1, -2, 0, 1, -3, 2, 5]
         for k, s in enumerate(specials):
             sint[k % nx, (k * 3) % ny] = s
-        mask[1, 1] = 0
-        index[0, 2] = 0           # aperture clipped at the track start
-        index[nx - 1, 3] = naz - 1  # ... and at the track end
-        index[2, 4] = -50         # centre before the first pulse
-        index[3, 5] = naz + 30    # centre after the last pulse
-        del flat_s
+        mask[1 % nx, 1 % ny] = 0
+        index[0, 2 % ny] = 0              # aperture clipped at the track start
+        index[nx - 1, 3 % ny] = naz - 1   # ... and at the track end
+        index[2 % nx, 4 % ny] = -50       # centre before the first pulse
+        index[3 % nx, 5 % ny] = naz + 30  # centre after the last pulse
     tgx = rng.uniform(xs[0], xs[-1], ntarget)
     tgy = rng.uniform(ys[0], ys[-1], ntarget)
     tgz = Z[np.clip(np.rint(tgx - xs[0]).astype(int), 0, nx - 1), np.clip(np.rint(tgy / 4).astype(int), 0, ny - 1)]
