@@ -115,6 +115,41 @@ int sarbp_focus(sarbp_dataset *ds, const double *x, const double *y, const doubl
                 const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
                 int64_t dimx, int precision, double *out_c128, int32_t *ncontrib);
 
+/* ---- many requests in one persistent launch ------------------------------------------------ */
+
+/* One request: a pixel set with its own grid shape and precision.  Inputs are host arrays of n
+ * elements (F-order flattened grid, dimx its fast dimension, 0 = plain list); they are copied at
+ * creation/submission.  tile_x: 0 = default. */
+typedef struct sarbp_request {
+  const double *x, *y, *z;
+  const uint8_t *mask;
+  const int32_t *index, *sint;
+  int64_t n;
+  int64_t dimx;
+  int32_t precision;
+  int32_t tile_x;
+} sarbp_request;
+
+typedef struct sarbp_batch sarbp_batch;
+
+/*
+ * Upload a set of requests against one dataset.  Has no counterpart in the reference beyond the
+ * pixel sub-batches of FocalizzatoreBpCuPy_Batched / process_dataset (gpu_small_dataset.py:285-336,
+ * main_gpu.py:519-560), which are launched one after another.
+ */
+int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nreq, sarbp_batch **out);
+/* Focus every request in ONE persistent launch.  tile_px: pixels per work item (64, 128 or 256:
+ * 1, 2 or 4 wavefronts of 64-pixel tiles); pulse_chunk: pulses of per-pulse coefficients staged
+ * in LDS per refill (64..1024, multiple of 64; tile_px/64 * pulse_chunk * 48 B <= 64 KiB).
+ * 0 for either uses the last autotuned (or default) value.  gpu_ms: kernel time (nullable). */
+int sarbp_batch_run(sarbp_batch *b, int32_t tile_px, int32_t pulse_chunk, float *gpu_ms);
+/* Time every valid (tile_px, pulse_chunk) in {64,128,256} x {64,128,256,512,1024} on this batch
+ * (reps launches each) and keep the fastest for later runs. */
+int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, int32_t *best_chunk,
+                         float *best_ms);
+int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t *ncontrib);
+int sarbp_batch_destroy(sarbp_batch *b);
+
 /*
  * Pixel-pulse pairs evaluated by a focusing pass (host-side, integers only): pixels with
  * mask == 1 and sint != -9999, pulses 0 < index + j - ceil(|sint|/2) < naz - 1 for

@@ -1,0 +1,335 @@
+// Host side of batched focusing: device buffers of requests, work-item lists for the persistent
+// kernel (bp_batch.hip), and the tile/pulse-chunk autotuner.
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <tuple>
+#include <vector>
+
+#include "batch_internal.h"
+
+namespace {
+constexpr size_t ALIGN = 256;
+size_t up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int64_t, int64_t, int64_t, int32_t>, std::pair<int, int>> g_tuned;  // -> (tile_px, chunk)
+}  // namespace
+
+int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b,
+                  hipStream_t st) {
+  b = ReqBuf();
+  if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
+  if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
+    return sarbp_fail(SARBP_EINVAL, "request pixel pointer is null");
+  if (r.dimx < 0 || (r.n > 0 && r.dimx > 0 && r.n % r.dimx))
+    return sarbp_fail(SARBP_EINVAL, "request pixel count must be a multiple of dimx");
+  if (r.precision != SARBP_PREC_F64 && r.precision != SARBP_PREC_SPLIT)
+    return sarbp_fail(SARBP_EINVAL, "unknown request precision");
+  const int tx = r.tile_x ? r.tile_x : default_tiling(r.precision).tile_x;
+  if (tx != 64 && tx != 32 && tx != 16 && tx != 8 && tx != 4) return sarbp_fail(SARBP_EINVAL, "bad tile_x");
+  b.n = r.n;
+  b.dimx = r.dimx > 0 ? r.dimx : (r.n > 0 ? r.n : 1);
+  b.dimy = r.n > 0 ? r.n / b.dimx : 0;
+  b.precision = r.precision;
+  b.accum = accum;
+  b.tg = make_tile_grid(b.dimx, b.dimy, tx);
+  const size_t n = (size_t)r.n;
+  size_t off[10];
+  size_t o = 0;
+  const size_t sz[9] = {n * 8, n * 8, n * 8, n, n * 4, n * 4, n * 16, n * 4, with_state ? n * sizeof(BpState) : 0};
+  for (int k = 0; k < 9; ++k) {
+    off[k] = o;
+    o += up(sz[k]);
+  }
+  off[9] = o;
+  b.bytes = (int64_t)o;
+  if (o > 0) {
+    hipError_t e = hipMalloc((void **)&b.base, o);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      b.base = nullptr;
+      return sarbp_fail(SARBP_ENOMEM, "request device allocation failed");
+    }
+  }
+  char *p = b.base;
+  b.x = (double *)(p + off[0]);
+  b.y = (double *)(p + off[1]);
+  b.z = (double *)(p + off[2]);
+  b.mask = (uint8_t *)(p + off[3]);
+  b.index = (int32_t *)(p + off[4]);
+  b.sint = (int32_t *)(p + off[5]);
+  b.out = (double2 *)(p + off[6]);
+  b.ncnt = (int32_t *)(p + off[7]);
+  b.state = with_state ? (BpState *)(p + off[8]) : nullptr;
+  const void *src[6] = {r.x, r.y, r.z, r.mask, r.index, r.sint};
+  void *dst[6] = {b.x, b.y, b.z, b.mask, b.index, b.sint};
+  for (int k = 0; k < 6 && n; ++k) {
+    hipError_t e = hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+      reqbuf_free(b);
+      return sarbp_hip_fail(e, "request upload");
+    }
+  }
+  // per-tile union lengths (host, integers only): work estimate for scheduling
+  const int64_t nt = b.tg.ntiles();
+  b.tile_len.assign((size_t)nt, 0);
+  if (n) {
+    std::vector<int> tlo((size_t)nt, INT_MAX), thi((size_t)nt, INT_MIN);
+    for (int64_t iy = 0; iy < b.dimy; ++iy)
+      for (int64_t ix = 0; ix < b.dimx; ++ix) {
+        const int64_t i = ix + b.dimx * iy;
+        if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
+        int64_t lo, hi;
+        sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, lo, hi);
+        if (hi <= lo) continue;
+        b.pp += hi - lo;
+        const int64_t t = (iy / b.tg.ty) * b.tg.ntx + ix / b.tg.tx;
+        tlo[(size_t)t] = std::min(tlo[(size_t)t], (int)lo);
+        thi[(size_t)t] = std::max(thi[(size_t)t], (int)hi);
+      }
+    for (int64_t t = 0; t < nt; ++t)
+      b.tile_len[(size_t)t] = thi[(size_t)t] > tlo[(size_t)t] ? thi[(size_t)t] - tlo[(size_t)t] : 0;
+  }
+  return SARBP_OK;
+}
+
+void reqbuf_free(ReqBuf &b) {
+  if (b.base) (void)hipFree(b.base);
+  b.base = nullptr;
+}
+
+BatchReqDev reqbuf_dev(const ReqBuf &b) {
+  BatchReqDev d;
+  d.px = PixelsView{b.x, b.y, b.z, b.mask, b.index, b.sint, b.n};
+  d.out = b.out;
+  d.ncnt = b.ncnt;
+  d.state = b.state;
+  d.tg = b.tg;
+  d.precision = b.precision;
+  d.accum = b.accum;
+  return d;
+}
+
+void build_queues(const std::vector<WorkItemCost> &work, std::vector<BatchItem> &items, std::vector<int32_t> &seg) {
+  // Longest first; then dealt to the queues in runs of K consecutive items, so each queue (XCD)
+  // works through neighbouring tiles and every queue sees the expensive items early.
+  std::vector<WorkItemCost> w(work);
+  std::stable_sort(w.begin(), w.end(), [](const WorkItemCost &a, const WorkItemCost &b) { return a.cost > b.cost; });
+  constexpr size_t K = 8;
+  std::vector<std::vector<BatchItem>> q(SARBP_BATCH_QUEUES);
+  for (size_t k = 0; k < w.size(); ++k) q[(k / K) % SARBP_BATCH_QUEUES].push_back(w[k].item);
+  items.clear();
+  seg.assign(SARBP_BATCH_QUEUES + 1, 0);
+  for (int qi = 0; qi < SARBP_BATCH_QUEUES; ++qi) {
+    seg[qi] = (int32_t)items.size();
+    items.insert(items.end(), q[qi].begin(), q[qi].end());
+  }
+  seg[SARBP_BATCH_QUEUES] = (int32_t)items.size();
+}
+
+int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
+                        const std::vector<int32_t> &seg, hipStream_t st) {
+  auto grow = [](void **p, size_t &cap, size_t need) -> hipError_t {
+    if (need <= cap) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(p, need);
+    if (e == hipSuccess) cap = need;
+    return e;
+  };
+  const size_t rb = std::max<size_t>(1, reqs.size()) * sizeof(BatchReqDev);
+  const size_t ib = std::max<size_t>(1, items.size()) * sizeof(BatchItem);
+  hipError_t e = grow((void **)&d_reqs, cap_reqs, rb);
+  if (e == hipSuccess) e = grow((void **)&d_items, cap_items, ib);
+  if (e == hipSuccess && !d_seg) e = hipMalloc((void **)&d_seg, (SARBP_BATCH_QUEUES + 1) * sizeof(int32_t));
+  if (e == hipSuccess && !d_heads) e = hipMalloc((void **)&d_heads, SARBP_BATCH_QUEUES * sizeof(int32_t));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return sarbp_fail(SARBP_ENOMEM, "work-list allocation failed");
+  }
+  if (!reqs.empty()) e = hipMemcpyAsync(d_reqs, reqs.data(), reqs.size() * sizeof(BatchReqDev), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(d_seg, seg.data(), seg.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "work-list upload");
+  nitems = (int)items.size();
+  return SARBP_OK;
+}
+
+void ItemBuffers::release() {
+  if (d_reqs) (void)hipFree(d_reqs);
+  if (d_items) (void)hipFree(d_items);
+  if (d_seg) (void)hipFree(d_seg);
+  if (d_heads) (void)hipFree(d_heads);
+  *this = ItemBuffers();
+}
+
+struct sarbp_batch {
+  sarbp_dataset *ds = nullptr;
+  std::vector<ReqBuf> reqs;
+  ItemBuffers ib;
+  int built_waves = 0;
+  int tile_px = 64, chunk = 64;
+  int64_t total_px = 0;
+};
+
+namespace {
+int batch_build(sarbp_batch *b, int waves) {
+  if (b->built_waves == waves) return SARBP_OK;
+  std::vector<BatchReqDev> devs;
+  std::vector<WorkItemCost> work;
+  for (size_t r = 0; r < b->reqs.size(); ++r) {
+    const ReqBuf &q = b->reqs[r];
+    devs.push_back(reqbuf_dev(q));
+    const int64_t nt = q.tg.ntiles();
+    for (int64_t t = 0; t < nt; t += waves) {
+      int64_t c = 0;
+      for (int64_t u = t; u < std::min(nt, t + waves); ++u) c = std::max<int64_t>(c, q.tile_len[(size_t)u]);
+      work.push_back(WorkItemCost{BatchItem{(int32_t)r, (int32_t)t, 0, INT32_MAX}, c});
+    }
+  }
+  std::vector<BatchItem> items;
+  std::vector<int32_t> seg;
+  build_queues(work, items, seg);
+  int rc = b->ib.upload(devs, items, seg, b->ds->stream);
+  if (rc) return rc;
+  b->built_waves = waves;
+  return SARBP_OK;
+}
+
+int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
+  const int waves = tile_px / 64;
+  int rc = batch_build(b, waves);
+  if (rc) return rc;
+  sarbp_dataset *ds = b->ds;
+  hipError_t e = hipEventRecord(ds->ev0, ds->stream);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
+  if (b->ib.nitems > 0) {
+    rc = sarbp_launch_batch(ds, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk, 0, ds->stream);
+    if (rc) return rc;
+  }
+  e = hipEventRecord(ds->ev1, ds->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ds->ev1);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "batch kernel");
+  if (ms) {
+    e = hipEventElapsedTime(ms, ds->ev0, ds->ev1);
+    if (e != hipSuccess) return sarbp_hip_fail(e, "event timing");
+  }
+  return SARBP_OK;
+}
+
+bool valid_combo(int tile_px, int chunk) {
+  if (tile_px != 64 && tile_px != 128 && tile_px != 256) return false;
+  if (chunk < 64 || chunk > 1024 || chunk % 64) return false;
+  return sarbp_batch_lds_bytes(tile_px / 64, chunk) <= 64 * 1024;
+}
+}  // namespace
+
+extern "C" {
+
+int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nreq, sarbp_batch **out) {
+  if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
+  *out = nullptr;
+  if (!ds) return sarbp_fail(SARBP_EINVAL, "dataset is null");
+  if (nreq < 0 || (nreq > 0 && !reqs)) return sarbp_fail(SARBP_EINVAL, "bad request list");
+  sarbp_batch *b = new (std::nothrow) sarbp_batch();
+  if (!b) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
+  b->ds = ds;
+  b->reqs.resize((size_t)nreq);
+  for (int32_t r = 0; r < nreq; ++r) {
+    int rc = reqbuf_create(ds, reqs[r], false, default_tiling(reqs[r].precision).accum, b->reqs[(size_t)r],
+                           ds->stream);
+    if (rc) {
+      sarbp_batch_destroy(b);
+      return rc;
+    }
+    b->total_px += reqs[r].n;
+  }
+  hipError_t e = hipStreamSynchronize(ds->stream);
+  if (e != hipSuccess) {
+    sarbp_batch_destroy(b);
+    return sarbp_hip_fail(e, "request upload");
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(std::make_tuple(ds->nra, ds->naz, b->total_px, nreq));
+    if (it != g_tuned.end()) {
+      b->tile_px = it->second.first;
+      b->chunk = it->second.second;
+    }
+  }
+  *out = b;
+  return SARBP_OK;
+}
+
+int sarbp_batch_run(sarbp_batch *b, int32_t tile_px, int32_t pulse_chunk, float *gpu_ms) {
+  if (!b) return sarbp_fail(SARBP_EINVAL, "null handle");
+  const int tp = tile_px ? tile_px : b->tile_px;
+  const int ch = pulse_chunk ? pulse_chunk : b->chunk;
+  if (!valid_combo(tp, ch)) return sarbp_fail(SARBP_EINVAL, "invalid tile_px / pulse_chunk combination");
+  return batch_time(b, tp, ch, gpu_ms);
+}
+
+int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, int32_t *best_chunk, float *best_ms) {
+  if (!b) return sarbp_fail(SARBP_EINVAL, "null handle");
+  if (reps < 1) reps = 1;
+  float best = 0.f;
+  int bt = 0, bc = 0;
+  for (int tp : {64, 128, 256})
+    for (int ch : {64, 128, 256, 512, 1024}) {
+      if (!valid_combo(tp, ch)) continue;
+      float m = 0.f;
+      for (int k = 0; k < reps; ++k) {
+        float t = 0.f;
+        int rc = batch_time(b, tp, ch, &t);
+        if (rc) return rc;
+        m = k == 0 ? t : std::min(m, t);
+      }
+      if (bt == 0 || m < best) {
+        best = m;
+        bt = tp;
+        bc = ch;
+      }
+    }
+  b->tile_px = bt;
+  b->chunk = bc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[std::make_tuple(b->ds->nra, b->ds->naz, b->total_px, (int32_t)b->reqs.size())] = {bt, bc};
+  }
+  if (best_tile_px) *best_tile_px = bt;
+  if (best_chunk) *best_chunk = bc;
+  if (best_ms) *best_ms = best;
+  return SARBP_OK;
+}
+
+int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t *ncontrib) {
+  if (!b) return sarbp_fail(SARBP_EINVAL, "null handle");
+  if (req < 0 || (size_t)req >= b->reqs.size()) return sarbp_fail(SARBP_EINVAL, "request index out of range");
+  const ReqBuf &q = b->reqs[(size_t)req];
+  hipStream_t st = b->ds->stream;
+  hipError_t e = hipSuccess;
+  if (out_c128 && q.n) e = hipMemcpyAsync(out_c128, q.out, (size_t)q.n * sizeof(double2), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && ncontrib && q.n)
+    e = hipMemcpyAsync(ncontrib, q.ncnt, (size_t)q.n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "batch download");
+  return SARBP_OK;
+}
+
+int sarbp_batch_destroy(sarbp_batch *b) {
+  if (!b) return SARBP_OK;
+  if (b->ds && b->ds->stream) (void)hipStreamSynchronize(b->ds->stream);
+  for (auto &q : b->reqs) reqbuf_free(q);
+  b->ib.release();
+  delete b;
+  return SARBP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+// Persistent batch kernel: many requests (pixel sets of uneven size, aperture length and
+// precision) focused in one launch against one device-resident dataset.
+//
+// The grid is sized to the device (blocks resident at once); each block repeatedly takes the next
+// work item from one of SARBP_BATCH_QUEUES queues -- its home queue (block % 8, the XCD the block
+// runs on) first, then the others -- until every queue is drained, so every block reaches the exit.
+// An item is W adjacent tiles of one request, one per wavefront; each wavefront focuses its tile
+// over the item's pulse range with the FP64 or split terms of bp_tile.h, starting from zero or from
+// the request's saved per-pixel state, and either writes the pixel means (last range of the tile)
+// or saves the state for a later launch.  The arithmetic per pixel is the same whatever the item
+// boundaries or the other requests in the launch, so a request's result does not depend on the mix.
+#include <climits>
+
+#include "bp_tile.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+__device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r, const BatchItem &it,
+                                           int64_t tile, PulseRec *rec, int C) {
+  const TileLane t = tile_lane(g, r.px, r.tg, tile, threadIdx.x & 63);
+  const bool first = it.p_off == 0;
+  if (t.wlo >= t.whi) {  // nothing to focus in this tile
+    if (first && t.valid) store_mean(r.out, r.ncnt, t.i, 0.0, 0.0, 0);
+    return;
+  }
+  const int p0 = t.wlo + it.p_off;
+  if (p0 >= t.whi) return;  // this tile finished in an earlier range
+  const int p1 = (it.p_end == INT32_MAX || it.p_end >= t.whi - t.wlo) ? t.whi : t.wlo + it.p_end;
+  const bool last = p1 >= t.whi;
+  double ar = 0.0, ai = 0.0;
+  float cr = 0.f, ci = 0.f;
+  int32_t cnt = 0;
+  if (!first && t.valid) {
+    const BpState st = r.state[t.i];
+    ar = st.ar;
+    ai = st.ai;
+    cr = st.cr;
+    ci = st.ci;
+    cnt = st.cnt;
+  }
+  bool split = r.precision == SARBP_PREC_SPLIT;
+  SplitTile s;
+  if (split) {
+    s = split_tile(g, t);
+    split = s.omax <= SPLIT_OMAX_M;  // not a compact tile: FP64 terms (same decision every range)
+  }
+  if (!split) {
+    f64_span(g, t, p0, p1, ar, ai, cnt);
+  } else if (r.accum == 0) {
+    float sr = (float)ar, si = (float)ai;  // the Kahan running sum is held exactly in the state
+    split_span<0>(g, t, s, p0, p1, rec, C, sr, si, cr, ci, ar, ai, cnt);
+    if (last) {
+      split_total<0>(sr, si, cr, ci, ar, ai);
+    } else {
+      ar = sr;
+      ai = si;
+    }
+  } else {
+    float sr = 0.f, si = 0.f;  // chunk sums are folded at every chunk end, so none is pending here
+    split_span<1>(g, t, s, p0, p1, rec, C, sr, si, cr, ci, ar, ai, cnt);
+  }
+  if (!t.valid) return;
+  if (last) {
+    store_mean(r.out, r.ncnt, t.i, ar, ai, cnt);
+  } else {
+    BpState st;
+    st.ar = ar;
+    st.ai = ai;
+    st.cr = cr;
+    st.ci = ci;
+    st.cnt = cnt;
+    st.pad = 0;
+    r.state[t.i] = st;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *__restrict__ reqs,
+                                                  const BatchItem *__restrict__ items,
+                                                  const int32_t *__restrict__ seg, int32_t *heads, int C) {
+  extern __shared__ PulseRec lds_rec[];
+  __shared__ int32_t next_item;
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  PulseRec *rec = lds_rec + wave * C;
+  const int home = blockIdx.x % SARBP_BATCH_QUEUES;
+  for (int qi = 0; qi < SARBP_BATCH_QUEUES;) {
+    const int q = (home + qi) % SARBP_BATCH_QUEUES;
+    if (threadIdx.x == 0) next_item = seg[q] + atomicAdd(&heads[q], 1);
+    __syncthreads();
+    const int32_t k = next_item;
+    __syncthreads();
+    if (k >= seg[q + 1]) {  // queue q drained: move on, never come back
+      ++qi;
+      continue;
+    }
+    const BatchItem it = items[k];
+    const BatchReqDev &r = reqs[it.req];
+    const int64_t tile = (int64_t)it.tile_lo + wave;
+    if (tile < r.tg.ntiles()) batch_tile(g, r, it, tile, rec, C);
+    (void)waves;
+  }
+}
+
+}  // namespace
+
+size_t sarbp_batch_lds_bytes(int waves, int chunk) { return (size_t)waves * (size_t)chunk * sizeof(PulseRec); }
+
+int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
+                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int grid_blocks,
+                       hipStream_t st) {
+  if (waves != 1 && waves != 2 && waves != 4) return sarbp_fail(SARBP_EINVAL, "waves per block must be 1, 2 or 4");
+  if (chunk < 64 || chunk > 1024 || chunk % 64) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 64..1024, x64");
+  const size_t lds = sarbp_batch_lds_bytes(waves, chunk);
+  if (lds > 64 * 1024) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk needs more than 64 KiB of LDS");
+  hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_BATCH_QUEUES * sizeof(int32_t), st);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "queue reset");
+  if (grid_blocks <= 0) {
+    int per_cu = 0, ncu = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bp_batch, waves * 64, lds);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ds->device);
+    if (e != hipSuccess) return sarbp_hip_fail(e, "occupancy query");
+    if (per_cu < 1) per_cu = 1;
+    grid_blocks = per_cu * ncu;
+  }
+  hipLaunchKernelGGL(k_bp_batch, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, ds->geom(), d_reqs, d_items,
+                     d_seg, d_heads, chunk);
+  e = hipGetLastError();
+  if (e != hipSuccess) return sarbp_hip_fail(e, "batch kernel launch");
+  return SARBP_OK;
+}

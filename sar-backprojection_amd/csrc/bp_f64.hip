@@ -14,7 +14,7 @@
 //                  scalar load and the RawData gathers of neighbouring pixels fall in the same or
 //                  adjacent range bins of one pulse column (few cache lines per load).
 // Both accumulate each pixel's terms in ascending pulse order, as the reference does.
-#include "bp_common.h"
+#include "bp_tile.h"
 
 #pragma clang fp contract(off)
 
@@ -44,38 +44,11 @@ __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, Til
                                                     int32_t *__restrict__ ncnt) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= tg.ntiles()) return;  // whole wavefront
-  const int lane = threadIdx.x;
-  const int64_t ix = (tile % tg.ntx) * tg.tx + lane % tg.tx;
-  const int64_t iy = (tile / tg.ntx) * tg.ty + lane / tg.tx;
-  const bool valid = ix < tg.dimx && iy < tg.dimy;
-  const int64_t i = ix + tg.dimx * iy;
-  double x = 0, y = 0, z = 0;
-  int lo = 0x7fffffff, hi = -0x7fffffff;  // empty aperture
-  if (valid) {
-    const int32_t s = px.sint[i];
-    if (px.mask[i] == 1 && s != -9999) {  // sar_focuser_puntual.py:46-53
-      int64_t l, h;
-      sarbp_pulse_range(px.index[i], s, g.naz, l, h);
-      if (h > l) {
-        lo = (int)l;
-        hi = (int)h;
-      }
-      x = px.x[i];
-      y = px.y[i];
-      z = px.z[i];
-    }
-  }
-  const int wlo = __builtin_amdgcn_readfirstlane(wave_min_i32(lo));
-  const int whi = __builtin_amdgcn_readfirstlane(wave_max_i32(hi));
-  double acc_re = 0.0, acc_im = 0.0;
+  const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
+  double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
-  for (int a = wlo; a < whi; ++a) {  // wave-uniform pulse
-    const double4 t = g.traj[a];     // scalar load
-    if (a >= lo && a < hi) {
-      if (f64_term(g, x, y, z, t.x, t.y, t.z, g.raw + g.nra * (int64_t)a, acc_re, acc_im)) ++cnt;
-    }
-  }
-  if (valid) store_mean(out, ncnt, i, acc_re, acc_im, cnt);
+  f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
+  if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 
 }  // namespace

@@ -1,0 +1,217 @@
+// Wavefront-per-tile building blocks shared by the scene kernels (bp_f64.hip, bp_split.hip) and
+// the persistent batch kernel (bp_batch.hip).
+//
+// A tile is up to 64 pixels of one request's F-order grid (tile_x along the fast axis); one
+// wavefront owns it, lane = pixel.  The wavefront walks the union [wlo, whi) of its pixels'
+// apertures pulse by pulse (the pulse index is wave-uniform), each lane adding only the pulses of
+// its own aperture, in ascending order as process_point does (sar_focuser_puntual.py:134-147).
+//
+// FP64 terms: bp_common.h.  Split terms (FP64 per tile and pulse, FP32 per pixel and pulse):
+//   per (tile, pulse), FP64:  D = Pc - T(p), R0 = |D| for the tile's reference point Pc (centre of
+//       its pixels' bounding box); the base phase phi(R0) = k*R0 + beta*R0^2 reduced to turns mod 1;
+//       the bin offset (R0 - near)/dra split into integer B0 and fraction f0; and the FP32
+//       coefficients u = D/R0, 1/(2R0), 1/(2R0^2), 1/(8R0^3), (k + 2 beta R0)/(2 pi), R0^2, 2R0.
+//       Lane l computes pulse c0+l of a 64-pulse chunk into the wavefront's LDS records.
+//   per (pixel, pulse), FP32:  with o = P - Pc and a = u.o, t2 = |o|^2 - a^2,
+//       dR = R - R0 = a + t2/(2R0) - a*t2/(2R0^2) [- t2(|o|^2 - 5a^2)/(8R0^3)]   (series of
+//       R0*(sqrt(1+e)-1), e = (2 R0 a + |o|^2)/R0^2, to 3rd, or beyond 12 m offsets 4th, order);
+//       bin = B0 + rint(f0 + dR/dra), re-decided in FP64 when within the error bound of a tie, so
+//       bins and contributor counts equal the FP64 path's; phase turns = t0 + fract(c1*dR) +
+//       (exact residual of that product) + beta/(2pi)*dR^2 into v_sin_f32 / v_cos_f32 (which take
+//       turns); weight R^2 = R0^2 + dR*(2R0 + dR); complex MAC in FP32 with Kahan compensation
+//       (ACCUM 0) or FP32 sums over each 64-pulse chunk folded into FP64 (ACCUM 1).
+//   A tile whose pixels lie more than SPLIT_OMAX_M from its centre (irregular pixel lists) takes the
+//   FP64 terms: the series and FP32 offsets are only accurate for compact tiles.
+#pragma once
+
+#include "bp_common.h"
+
+namespace {
+
+constexpr float SPLIT_OMAX_M = 48.0f;    // largest pixel-to-centre offset focused in split form
+constexpr float SPLIT_FOURTH_M = 12.0f;  // include the 4th-order range term beyond this offset
+constexpr double INV_2PI = 0.15915494309189533577;
+
+// Tile of one pixel grid, as seen by one lane.
+struct TileLane {
+  int64_t i = 0;       // flat pixel index
+  bool valid = false;  // lane maps to a pixel of the grid
+  double x = 0, y = 0, z = 0;
+  int lo = 0x7fffffff, hi = -0x7fffffff;  // own pulses [lo, hi); empty when lo >= hi
+  int wlo = 0, whi = 0;                   // union over the wavefront (uniform)
+};
+
+__device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView &px, const TileGrid &tg,
+                                              int64_t tile, int lane) {
+  TileLane t;
+  const int64_t ix = (tile % tg.ntx) * tg.tx + lane % tg.tx;
+  const int64_t iy = (tile / tg.ntx) * tg.ty + lane / tg.tx;
+  t.valid = ix < tg.dimx && iy < tg.dimy;
+  t.i = ix + tg.dimx * iy;
+  if (t.valid) {
+    const int32_t s = px.sint[t.i];
+    if (px.mask[t.i] == 1 && s != -9999) {  // sar_focuser_puntual.py:46-53
+      int64_t l, h;
+      sarbp_pulse_range(px.index[t.i], s, g.naz, l, h);
+      if (h > l) {
+        t.lo = (int)l;
+        t.hi = (int)h;
+      }
+      t.x = px.x[t.i];
+      t.y = px.y[t.i];
+      t.z = px.z[t.i];
+    }
+  }
+  t.wlo = __builtin_amdgcn_readfirstlane(wave_min_i32(t.lo));
+  t.whi = __builtin_amdgcn_readfirstlane(wave_max_i32(t.hi));
+  return t;
+}
+
+// FP64 terms for pulses [p0, p1) of the union.
+__device__ __forceinline__ void f64_span(const BpGeom &g, const TileLane &t, int p0, int p1, double &ar,
+                                         double &ai, int32_t &cnt) {
+  for (int a = p0; a < p1; ++a) {  // wave-uniform pulse
+    const double4 tr = g.traj[a];  // scalar load
+    if (a >= t.lo && a < t.hi) {
+      if (f64_term(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, g.raw + g.nra * (int64_t)a, ar, ai)) ++cnt;
+    }
+  }
+}
+
+struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads)
+  float4 a;        // ux, uy, uz, 1/(2R0)
+  float4 b;        // 1/(2R0^2), f0 = q0 - B0, B0 (int bits), t0 = frac(phi0/2pi)
+  float4 c;        // c1 = (k + 2 beta R0)/2pi, R0^2, 2R0, 1/(8R0^3)
+};
+
+__device__ __forceinline__ void make_rec(const BpGeom &g, double xc, double yc, double zc, double4 t,
+                                         PulseRec &r) {
+  const double dx = xc - t.x, dy = yc - t.y, dz = zc - t.z;
+  const double r0sq = dx * dx + dy * dy + dz * dz;
+  const double r0 = __builtin_sqrt(r0sq);
+  const double inv = 1.0 / r0;
+  const double q0 = (r0 - g.near_range) * g.inv_dra;
+  const double b0 = __builtin_rint(q0);
+  const double ph0 = g.kphase * r0 + g.beta * r0sq;
+  const double turns = ph0 * INV_2PI;
+  const double t0 = turns - __builtin_floor(turns);
+  const double c1 = (g.kphase + 2.0 * g.beta * r0) * INV_2PI;
+  r.a = make_float4((float)(dx * inv), (float)(dy * inv), (float)(dz * inv), (float)(0.5 * inv));
+  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float((int)b0), (float)t0);
+  r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
+}
+
+// Tile-relative quantities of the split form (uniform except the offsets).
+struct SplitTile {
+  double xc, yc, zc;  // reference point
+  float ox, oy, oz, qo;
+  float omax;         // largest pixel-to-centre distance (uniform)
+  float guard;        // |f0 + dR/dra - nearest integer| above this -> decide the bin in FP64
+};
+
+__device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileLane &t) {
+  SplitTile s;
+  const bool act = t.hi > t.lo;
+  const double inf = __builtin_huge_val();
+  s.xc = 0.5 * (wave_min_f64(act ? t.x : inf) + wave_max_f64(act ? t.x : -inf));
+  s.yc = 0.5 * (wave_min_f64(act ? t.y : inf) + wave_max_f64(act ? t.y : -inf));
+  s.zc = 0.5 * (wave_min_f64(act ? t.z : inf) + wave_max_f64(act ? t.z : -inf));
+  s.ox = act ? (float)(t.x - s.xc) : 0.f;
+  s.oy = act ? (float)(t.y - s.yc) : 0.f;
+  s.oz = act ? (float)(t.z - s.zc) : 0.f;
+  s.qo = s.ox * s.ox + s.oy * s.oy + s.oz * s.oz;
+  s.omax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sqrtf((float)wave_max_f64((double)s.qo)))));
+  // error of f0 + dR/dra in bins: f0 and the FP32 sum ~1e-7, dR ~1.2e-7 * |o| m; margin >= 3x
+  s.guard = 0.5f - (2e-5f + 4e-7f * s.omax * (float)g.inv_dra);
+  return s;
+}
+
+// Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
+// and staged records do not depend on where a caller splits the range.  rec: this wavefront's C
+// LDS records (C a multiple of 64; lane l stages pulses c0 + l + 64*m).  Sums: Kahan
+// (sr, si, cr, ci) for ACCUM 0; for ACCUM 1 (sr, si) hold the running chunk sum, folded into
+// (ar, ai) at each chunk end.
+template <int ACCUM>
+__device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
+                                           int p1, PulseRec *rec, int C, float &sr, float &si, float &cr,
+                                           float &ci, double &ar, double &ai, int32_t &cnt) {
+  const int lane = threadIdx.x & 63;
+  const bool fourth = s.omax > SPLIT_FOURTH_M;
+  const float inv_dra = (float)g.inv_dra;
+  const float beta_t = (float)(g.beta * INV_2PI);
+  const unsigned nbins = (unsigned)(g.nra - 1);
+  for (; c0 < p1; c0 += C) {
+    for (int m = lane; m < C; m += 64) {
+      const int p = c0 + m;
+      if (p < p1) make_rec(g, s.xc, s.yc, s.zc, g.traj[p], rec[m]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int cn = min(C, p1 - c0);
+    for (int k = 0; k < cn; ++k) {
+      const int a = c0 + k;
+      const float4 ra = rec[k].a, rb = rec[k].b, rc = rec[k].c;
+      if (a >= t.lo && a < t.hi) {
+        const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
+        const float t2 = fmaf(-av, av, s.qo);
+        float dR = fmaf(t2, ra.w, av);
+        dR = fmaf(-(av * t2), rb.x, dR);
+        if (fourth) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
+        const float qb = fmaf(dR, inv_dra, rb.y);
+        const float rq = __builtin_rintf(qb);
+        int bin;
+        if (fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly
+          const double4 tr = g.traj[a];
+          bin = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);
+        } else {
+          bin = __float_as_int(rb.z) + (int)rq;
+        }
+        if ((unsigned)bin < nbins) {
+          const float2 d = g.raw[(int64_t)a * g.nra + bin];
+          const float ph = rc.x * dR;
+          const float phl = fmaf(rc.x, dR, -ph);  // exact residual of the product
+          const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, rb.w) + phl);
+          const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
+          const float cs = __builtin_amdgcn_cosf(tt);
+          const float w = fmaf(dR, rc.z + dR, rc.y);  // R^2
+          const float wr = w * d.x, wi = w * d.y;
+          const float tre = fmaf(wr, cs, wi * sn);    // R^2 d exp(-j phi)
+          const float tim = fmaf(wi, cs, -(wr * sn));
+          if (ACCUM == 0) {
+            const float yr = tre - cr, yi = tim - ci;
+            const float nr = sr + yr, ni = si + yi;
+            cr = (nr - sr) - yr;
+            ci = (ni - si) - yi;
+            sr = nr;
+            si = ni;
+          } else {
+            sr += tre;
+            si += tim;
+          }
+          ++cnt;
+        }
+      }
+    }
+    if (ACCUM == 1) {
+      ar += (double)sr;
+      ai += (double)si;
+      sr = 0.f;
+      si = 0.f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();  // reads of this chunk finish before the next overwrites rec
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// Final sums of the split form as FP64.
+template <int ACCUM>
+__device__ __forceinline__ void split_total(float sr, float si, float cr, float ci, double &ar, double &ai) {
+  if (ACCUM == 0) {
+    ar = (double)sr - (double)cr;
+    ai = (double)si - (double)ci;
+  }
+}
+
+}  // namespace

@@ -96,6 +96,8 @@ __host__ __device__ inline void sarbp_pulse_range(int32_t index, int32_t sint, i
 
 Tiling default_tiling(int precision);
 
+
+
 // error plumbing (api.hip)
 int sarbp_fail(int code, const std::string &msg);
 int sarbp_hip_fail(hipError_t e, const char *what);
@@ -154,3 +156,44 @@ __device__ inline double wave_max_f64(double v) {
   for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
   return v;
 }
+
+// ---- batched / served focusing: many pixel sets (requests) in one persistent launch ----------
+
+// Partial focusing state of one pixel between launches (32 B): FP64 sums (the Kahan running sum
+// of the split path is held exactly in ar/ai), Kahan compensations, contributor count.
+struct BpState {
+  double ar, ai;
+  float cr, ci;
+  int32_t cnt, pad;
+};
+
+// One request as the batch kernel sees it (device pointers).
+struct BatchReqDev {
+  PixelsView px;
+  double2 *out;
+  int32_t *ncnt;
+  BpState *state;    // partial sums between launches (serving); may be null for one-shot batches
+  TileGrid tg;
+  int32_t precision;  // SARBP_PREC_*
+  int32_t accum;      // split path accumulation (0 Kahan, 1 FP64 fold)
+};
+
+// Work item: tiles [tile_lo, tile_lo + W) of one request (one per wavefront of the block), pulses
+// [wlo + p_off, wlo + p_end) of each tile's union (p_end = INT32_MAX: to the end).  p_off and a
+// finite p_end are multiples of the pulse chunk C.
+struct BatchItem {
+  int32_t req;
+  int32_t tile_lo;
+  int32_t p_off;
+  int32_t p_end;
+};
+
+constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
+
+// Launch the persistent batch kernel: `waves` wavefronts per block (1, 2 or 4), pulse chunk C.
+// d_heads: SARBP_BATCH_QUEUES ints, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item
+// offsets.  grid_blocks <= 0 sizes the grid from the occupancy.
+int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
+                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int grid_blocks,
+                       hipStream_t st);
+size_t sarbp_batch_lds_bytes(int waves, int chunk);

@@ -32,6 +32,17 @@ _pp = ctypes.POINTER(ctypes.c_void_p)
 
 FILL_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p)
 
+
+class Request(ctypes.Structure):
+    """sarbp_request (include/sarbp.h)."""
+    _fields_ = [("x", _p), ("y", _p), ("z", _p), ("mask", _p), ("index", _p), ("sint", _p),
+                ("n", _i64), ("dimx", _i64), ("precision", _i32), ("tile_x", _i32)]
+
+
+_RP = ctypes.POINTER(Request)
+_i32p = ctypes.POINTER(_i32)
+_fp = ctypes.POINTER(ctypes.c_float)
+
 # name -> (restype, argtypes); every symbol include/sarbp.h declares.
 SIGNATURES = {
     "sarbp_abi_version": (ctypes.c_int, []),
@@ -49,6 +60,11 @@ SIGNATURES = {
     "sarbp_scene_download": (ctypes.c_int, [_p, _p, _p]),
     "sarbp_focus": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, ctypes.c_int, _p, _p]),
     "sarbp_count_pp": (_i64, [_p, _p, _p, _i64, _i64]),
+    "sarbp_batch_create": (ctypes.c_int, [_p, _RP, _i32, _pp]),
+    "sarbp_batch_run": (ctypes.c_int, [_p, _i32, _i32, _fp]),
+    "sarbp_batch_autotune": (ctypes.c_int, [_p, _i32, _i32p, _i32p, _fp]),
+    "sarbp_batch_download": (ctypes.c_int, [_p, _i32, _p, _p]),
+    "sarbp_batch_destroy": (ctypes.c_int, [_p]),
 }
 
 _lib = None
