@@ -139,12 +139,13 @@ typedef struct sarbp_batch sarbp_batch;
  */
 int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nreq, sarbp_batch **out);
 /* Focus every request in ONE persistent launch.  tile_px: pixels per work item (64, 128 or 256:
- * 1, 2 or 4 wavefronts of 64-pixel tiles); pulse_chunk: pulses of per-pulse coefficients staged
- * in LDS per refill (64..1024, multiple of 64; tile_px/64 * pulse_chunk * 48 B <= 64 KiB).
- * 0 for either uses the last autotuned (or default) value.  gpu_ms: kernel time (nullable). */
+ * 1, 2 or 4 wavefronts of 64-pixel tiles); pulse_chunk: pulses of per-(pulse, tile row)
+ * coefficients staged in LDS per refill (8..1024, multiple of 8; tile_px/64 * pulse_chunk *
+ * rows per tile * 48 B <= 64 KiB).  0 for either uses the last autotuned (or default) value.
+ * The results do not depend on tile_px or pulse_chunk.  gpu_ms: kernel time (nullable). */
 int sarbp_batch_run(sarbp_batch *b, int32_t tile_px, int32_t pulse_chunk, float *gpu_ms);
-/* Time every valid (tile_px, pulse_chunk) in {64,128,256} x {64,128,256,512,1024} on this batch
- * (reps launches each) and keep the fastest for later runs. */
+/* Time every valid (tile_px, pulse_chunk) in {64,128,256} x {16,32,...,1024} on this batch
+ * (reps launches each, best kept) and use the fastest for later runs of batches of this shape. */
 int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, int32_t *best_chunk,
                          float *best_ms);
 int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t *ncontrib);

@@ -172,10 +172,11 @@ void ItemBuffers::release() {
 
 struct sarbp_batch {
   sarbp_dataset *ds = nullptr;
+  int max_rows = 1;
   std::vector<ReqBuf> reqs;
   ItemBuffers ib;
   int built_waves = 0;
-  int tile_px = 64, chunk = 64;
+  int tile_px = 64, chunk = 32;
   int64_t total_px = 0;
 };
 
@@ -211,7 +212,8 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
   if (b->ib.nitems > 0) {
-    rc = sarbp_launch_batch(ds, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk, 0, ds->stream);
+    rc = sarbp_launch_batch(ds, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk, b->max_rows,
+                            0, ds->stream);
     if (rc) return rc;
   }
   e = hipEventRecord(ds->ev1, ds->stream);
@@ -224,10 +226,10 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   return SARBP_OK;
 }
 
-bool valid_combo(int tile_px, int chunk) {
+bool valid_combo(int tile_px, int chunk, int max_rows) {
   if (tile_px != 64 && tile_px != 128 && tile_px != 256) return false;
-  if (chunk < 64 || chunk > 1024 || chunk % 64) return false;
-  return sarbp_batch_lds_bytes(tile_px / 64, chunk) <= 64 * 1024;
+  if (chunk < 8 || chunk > 1024 || chunk % 8) return false;
+  return sarbp_batch_lds_bytes(tile_px / 64, chunk, max_rows) <= 64 * 1024;
 }
 }  // namespace
 
@@ -250,6 +252,7 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
       return rc;
     }
     b->total_px += reqs[r].n;
+    b->max_rows = std::max(b->max_rows, b->reqs[(size_t)r].tg.ty);
   }
   hipError_t e = hipStreamSynchronize(ds->stream);
   if (e != hipSuccess) {
@@ -272,7 +275,7 @@ int sarbp_batch_run(sarbp_batch *b, int32_t tile_px, int32_t pulse_chunk, float 
   if (!b) return sarbp_fail(SARBP_EINVAL, "null handle");
   const int tp = tile_px ? tile_px : b->tile_px;
   const int ch = pulse_chunk ? pulse_chunk : b->chunk;
-  if (!valid_combo(tp, ch)) return sarbp_fail(SARBP_EINVAL, "invalid tile_px / pulse_chunk combination");
+  if (!valid_combo(tp, ch, b->max_rows)) return sarbp_fail(SARBP_EINVAL, "invalid tile_px / pulse_chunk combination");
   return batch_time(b, tp, ch, gpu_ms);
 }
 
@@ -282,8 +285,8 @@ int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, in
   float best = 0.f;
   int bt = 0, bc = 0;
   for (int tp : {64, 128, 256})
-    for (int ch : {64, 128, 256, 512, 1024}) {
-      if (!valid_combo(tp, ch)) continue;
+    for (int ch : {16, 32, 64, 128, 256, 512, 1024}) {
+      if (!valid_combo(tp, ch, b->max_rows)) continue;
       float m = 0.f;
       for (int k = 0; k < reps; ++k) {
         float t = 0.f;

@@ -43,7 +43,7 @@ __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r
   bool split = r.precision == SARBP_PREC_SPLIT;
   SplitTile s;
   if (split) {
-    s = split_tile(g, t);
+    s = split_tile(g, r.tg, t);
     split = s.omax <= SPLIT_OMAX_M;  // not a compact tile: FP64 terms (same decision every range)
   }
   if (!split) {
@@ -76,14 +76,16 @@ __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r
   }
 }
 
+// C pulses of records per refill, up to max_rows rows per tile: C * max_rows records per wavefront.
 __global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *__restrict__ reqs,
                                                   const BatchItem *__restrict__ items,
-                                                  const int32_t *__restrict__ seg, int32_t *heads, int C) {
+                                                  const int32_t *__restrict__ seg, int32_t *heads, int C,
+                                                  int max_rows) {
   extern __shared__ PulseRec lds_rec[];
   __shared__ int32_t next_item;
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  PulseRec *rec = lds_rec + wave * C;
+  PulseRec *rec = lds_rec + wave * C * max_rows;
   const int home = blockIdx.x % SARBP_BATCH_QUEUES;
   for (int qi = 0; qi < SARBP_BATCH_QUEUES;) {
     const int q = (home + qi) % SARBP_BATCH_QUEUES;
@@ -105,15 +107,18 @@ __global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *_
 
 }  // namespace
 
-size_t sarbp_batch_lds_bytes(int waves, int chunk) { return (size_t)waves * (size_t)chunk * sizeof(PulseRec); }
+size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
+  return (size_t)waves * (size_t)chunk * (size_t)max_rows * sizeof(PulseRec);
+}
 
 int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
-                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int grid_blocks,
+                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks,
                        hipStream_t st) {
   if (waves != 1 && waves != 2 && waves != 4) return sarbp_fail(SARBP_EINVAL, "waves per block must be 1, 2 or 4");
-  if (chunk < 64 || chunk > 1024 || chunk % 64) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 64..1024, x64");
-  const size_t lds = sarbp_batch_lds_bytes(waves, chunk);
-  if (lds > 64 * 1024) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk needs more than 64 KiB of LDS");
+  if (chunk < 8 || chunk > 1024 || chunk % 8) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 8..1024, x8");
+  if (max_rows < 1 || max_rows > 64) return sarbp_fail(SARBP_EINVAL, "bad tile row count");
+  const size_t lds = sarbp_batch_lds_bytes(waves, chunk, max_rows);
+  if (lds > 64 * 1024) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk x tile rows needs more than 64 KiB of LDS");
   hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_BATCH_QUEUES * sizeof(int32_t), st);
   if (e != hipSuccess) return sarbp_hip_fail(e, "queue reset");
   if (grid_blocks <= 0) {
@@ -125,7 +130,7 @@ int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const
     grid_blocks = per_cu * ncu;
   }
   hipLaunchKernelGGL(k_bp_batch, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, ds->geom(), d_reqs, d_items,
-                     d_seg, d_heads, chunk);
+                     d_seg, d_heads, chunk, max_rows);
   e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "batch kernel launch");
   return SARBP_OK;

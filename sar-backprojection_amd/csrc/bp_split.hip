@@ -5,22 +5,26 @@
 
 namespace {
 
+// LDS records staged per refill: C pulses x TY rows = SPLIT_SCENE_RECS (6 KiB per wavefront).
+constexpr int SPLIT_SCENE_RECS = 128;
+
 template <int ACCUM>
 __global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
                                                       double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
-  __shared__ PulseRec rec[64];
+  __shared__ PulseRec rec[SPLIT_SCENE_RECS];
+  const int C = SPLIT_SCENE_RECS / tg.ty;
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= tg.ntiles()) return;  // whole wavefront
   const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
   double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
   if (t.wlo < t.whi) {
-    const SplitTile s = split_tile(g, t);
+    const SplitTile s = split_tile(g, tg, t);
     if (s.omax > SPLIT_OMAX_M) {  // not a compact tile: FP64 terms
       f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
     } else {
       float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
-      split_span<ACCUM>(g, t, s, t.wlo, t.whi, rec, 64, sr, si, cr, ci, ar, ai, cnt);
+      split_span<ACCUM>(g, t, s, t.wlo, t.whi, rec, C, sr, si, cr, ci, ar, ai, cnt);
       split_total<ACCUM>(sr, si, cr, ci, ar, ai);
     }
   }

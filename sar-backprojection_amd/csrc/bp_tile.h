@@ -7,11 +7,11 @@
 // its own aperture, in ascending order as process_point does (sar_focuser_puntual.py:134-147).
 //
 // FP64 terms: bp_common.h.  Split terms (FP64 per tile and pulse, FP32 per pixel and pulse):
-//   per (tile, pulse), FP64:  D = Pc - T(p), R0 = |D| for the tile's reference point Pc (centre of
-//       its pixels' bounding box); the base phase phi(R0) = k*R0 + beta*R0^2 reduced to turns mod 1;
+//   per (tile row, pulse), FP64:  D = Pc - T(p), R0 = |D| for the row's reference point Pc (centre
+//       of the bounding box of the row's pixels); the base phase phi(R0) = k*R0 + beta*R0^2 reduced to turns mod 1;
 //       the bin offset (R0 - near)/dra split into integer B0 and fraction f0; and the FP32
 //       coefficients u = D/R0, 1/(2R0), 1/(2R0^2), 1/(8R0^3), (k + 2 beta R0)/(2 pi), R0^2, 2R0.
-//       Lane l computes pulse c0+l of a 64-pulse chunk into the wavefront's LDS records.
+//       Lanes stage the records of a chunk of pulses (x rows) into the wavefront's LDS.
 //   per (pixel, pulse), FP32:  with o = P - Pc and a = u.o, t2 = |o|^2 - a^2,
 //       dR = R - R0 = a + t2/(2R0) - a*t2/(2R0^2) [- t2(|o|^2 - 5a^2)/(8R0^3)]   (series of
 //       R0*(sqrt(1+e)-1), e = (2 R0 a + |o|^2)/R0^2, to 3rd, or beyond 12 m offsets 4th, order);
@@ -101,24 +101,48 @@ __device__ __forceinline__ void make_rec(const BpGeom &g, double xc, double yc, 
   r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
 }
 
-// Tile-relative quantities of the split form (uniform except the offsets).
+// Row-relative quantities of the split form.  Each row of the tile (TX lanes at one grid y) has
+// its own reference point, the centre of the bounding box of its focusing pixels, so the FP32
+// offsets stay within half a tile width along the fast axis; records are staged per (pulse, row).
 struct SplitTile {
-  double xc, yc, zc;  // reference point
-  float ox, oy, oz, qo;
-  float omax;         // largest pixel-to-centre distance (uniform)
-  float guard;        // |f0 + dR/dra - nearest integer| above this -> decide the bin in FP64
+  double sxc, syc, szc;  // reference point of the row this lane stages records for (row lane % TY)
+  float ox, oy, oz, qo;  // offset of this lane's pixel from its row's reference point
+  float omax;            // largest pixel-to-reference distance in the tile (uniform)
+  float guard;           // |f0 + dR/dra - nearest integer| above this -> decide the bin in FP64
+  int ty_shift;          // log2(rows per tile)
+  int row;               // this lane's row
 };
 
-__device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileLane &t) {
+__device__ __forceinline__ double row_min_f64(double v, int tx) {
+  for (int m = tx >> 1; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ double row_max_f64(double v, int tx) {
+  for (int m = tx >> 1; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+__device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid &tg, const TileLane &t) {
   SplitTile s;
+  const int lane = threadIdx.x & 63;
   const bool act = t.hi > t.lo;
   const double inf = __builtin_huge_val();
-  s.xc = 0.5 * (wave_min_f64(act ? t.x : inf) + wave_max_f64(act ? t.x : -inf));
-  s.yc = 0.5 * (wave_min_f64(act ? t.y : inf) + wave_max_f64(act ? t.y : -inf));
-  s.zc = 0.5 * (wave_min_f64(act ? t.z : inf) + wave_max_f64(act ? t.z : -inf));
-  s.ox = act ? (float)(t.x - s.xc) : 0.f;
-  s.oy = act ? (float)(t.y - s.yc) : 0.f;
-  s.oz = act ? (float)(t.z - s.zc) : 0.f;
+  const double lx = row_min_f64(act ? t.x : inf, tg.tx), hx = row_max_f64(act ? t.x : -inf, tg.tx);
+  const double ly = row_min_f64(act ? t.y : inf, tg.tx), hy = row_max_f64(act ? t.y : -inf, tg.tx);
+  const double lz = row_min_f64(act ? t.z : inf, tg.tx), hz = row_max_f64(act ? t.z : -inf, tg.tx);
+  const bool row_empty = !(lx <= hx);
+  const double xc = row_empty ? 0.0 : 0.5 * (lx + hx);  // rows without pixels: any finite point
+  const double yc = row_empty ? 0.0 : 0.5 * (ly + hy);
+  const double zc = row_empty ? 0.0 : 0.5 * (lz + hz);
+  s.row = lane / tg.tx;
+  s.ty_shift = __builtin_ctz((unsigned)tg.ty);
+  const int src = (lane & (tg.ty - 1)) * tg.tx;  // first lane of the row this lane stages for
+  s.sxc = __shfl(xc, src, 64);
+  s.syc = __shfl(yc, src, 64);
+  s.szc = __shfl(zc, src, 64);
+  s.ox = act ? (float)(t.x - xc) : 0.f;
+  s.oy = act ? (float)(t.y - yc) : 0.f;
+  s.oz = act ? (float)(t.z - zc) : 0.f;
   s.qo = s.ox * s.ox + s.oy * s.oy + s.oz * s.oz;
   s.omax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sqrtf((float)wave_max_f64((double)s.qo)))));
   // error of f0 + dR/dra in bins: f0 and the FP32 sum ~1e-7, dR ~1.2e-7 * |o| m; margin >= 3x
@@ -127,10 +151,10 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileLane 
 }
 
 // Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
-// and staged records do not depend on where a caller splits the range.  rec: this wavefront's C
-// LDS records (C a multiple of 64; lane l stages pulses c0 + l + 64*m).  Sums: Kahan
-// (sr, si, cr, ci) for ACCUM 0; for ACCUM 1 (sr, si) hold the running chunk sum, folded into
-// (ar, ai) at each chunk end.
+// and staged records do not depend on where a caller splits the range.  rec: this wavefront's
+// C * TY LDS records, record k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
+// stages records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for
+// ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
 template <int ACCUM>
 __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                            int p1, PulseRec *rec, int C, float &sr, float &si, float &cr,
@@ -140,10 +164,12 @@ __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, c
   const float inv_dra = (float)g.inv_dra;
   const float beta_t = (float)(g.beta * INV_2PI);
   const unsigned nbins = (unsigned)(g.nra - 1);
+  const int nrec = C << s.ty_shift;
+  const PulseRec *my = rec + s.row;
   for (; c0 < p1; c0 += C) {
-    for (int m = lane; m < C; m += 64) {
-      const int p = c0 + m;
-      if (p < p1) make_rec(g, s.xc, s.yc, s.zc, g.traj[p], rec[m]);
+    for (int m = lane; m < nrec; m += 64) {
+      const int p = c0 + (m >> s.ty_shift);
+      if (p < p1) make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[m]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -151,7 +177,8 @@ __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, c
     const int cn = min(C, p1 - c0);
     for (int k = 0; k < cn; ++k) {
       const int a = c0 + k;
-      const float4 ra = rec[k].a, rb = rec[k].b, rc = rec[k].c;
+      const PulseRec *rk = my + (k << s.ty_shift);
+      const float4 ra = rk->a, rb = rk->b, rc = rk->c;
       if (a >= t.lo && a < t.hi) {
         const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
         const float t2 = fmaf(-av, av, s.qo);

@@ -190,10 +190,11 @@ struct BatchItem {
 
 constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
 
-// Launch the persistent batch kernel: `waves` wavefronts per block (1, 2 or 4), pulse chunk C.
+// Launch the persistent batch kernel: `waves` wavefronts per block (1, 2 or 4), pulse chunk C,
+// tiles of at most max_rows rows (LDS holds C * max_rows pulse records per wavefront).
 // d_heads: SARBP_BATCH_QUEUES ints, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item
 // offsets.  grid_blocks <= 0 sizes the grid from the occupancy.
 int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
-                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int grid_blocks,
+                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks,
                        hipStream_t st);
-size_t sarbp_batch_lds_bytes(int waves, int chunk);
+size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);
