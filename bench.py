@@ -91,8 +91,8 @@ def main():
     ap.add_argument("--strip", type=int, default=256, help="pixel columns of the CPU baseline strip")
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
-    ap.add_argument("--f64-map", default="1,16", help="FP64 work mapping kernel,tile_x")
-    ap.add_argument("--split-map", default="16,0", help="split work mapping tile_x,accum")
+    ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
+    ap.add_argument("--split-map", default="32,0", help="split work mapping tile_x,accum")
     ap.add_argument("--sweep", action="store_true", help="time every work mapping first and use the fastest")
     args = ap.parse_args()
     if args.gpus != 1:

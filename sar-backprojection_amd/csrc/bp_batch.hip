@@ -18,7 +18,7 @@
 namespace {
 
 __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r, const BatchItem &it,
-                                           int64_t tile, PulseRec *rec, int C) {
+                                           int64_t tile, int rec0, int C) {
   const TileLane t = tile_lane(g, r.px, r.tg, tile, threadIdx.x & 63);
   const bool first = it.p_off == 0;
   if (t.wlo >= t.whi) {  // nothing to focus in this tile
@@ -50,7 +50,7 @@ __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r
     f64_span(g, t, p0, p1, ar, ai, cnt);
   } else if (r.accum == 0) {
     float sr = (float)ar, si = (float)ai;  // the Kahan running sum is held exactly in the state
-    split_span<0>(g, t, s, p0, p1, rec, C, sr, si, cr, ci, ar, ai, cnt);
+    split_span<0>(g, t, s, p0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
     if (last) {
       split_total<0>(sr, si, cr, ci, ar, ai);
     } else {
@@ -59,7 +59,7 @@ __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r
     }
   } else {
     float sr = 0.f, si = 0.f;  // chunk sums are folded at every chunk end, so none is pending here
-    split_span<1>(g, t, s, p0, p1, rec, C, sr, si, cr, ci, ar, ai, cnt);
+    split_span<1>(g, t, s, p0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
   }
   if (!t.valid) return;
   if (last) {
@@ -81,11 +81,10 @@ __global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *_
                                                   const BatchItem *__restrict__ items,
                                                   const int32_t *__restrict__ seg, int32_t *heads, int C,
                                                   int max_rows) {
-  extern __shared__ PulseRec lds_rec[];
   __shared__ int32_t next_item;
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  PulseRec *rec = lds_rec + wave * C * max_rows;
+  const int rec0 = wave * C * max_rows;  // this wavefront's records in sarbp_lds_rec
   const int home = blockIdx.x % SARBP_BATCH_QUEUES;
   for (int qi = 0; qi < SARBP_BATCH_QUEUES;) {
     const int q = (home + qi) % SARBP_BATCH_QUEUES;
@@ -100,7 +99,7 @@ __global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *_
     const BatchItem it = items[k];
     const BatchReqDev &r = reqs[it.req];
     const int64_t tile = (int64_t)it.tile_lo + wave;
-    if (tile < r.tg.ntiles()) batch_tile(g, r, it, tile, rec, C);
+    if (tile < r.tg.ntiles()) batch_tile(g, r, it, tile, rec0, C);
     (void)waves;
   }
 }

@@ -11,8 +11,7 @@ constexpr int SPLIT_SCENE_RECS = 128;
 template <int ACCUM>
 __global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
                                                       double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
-  __shared__ PulseRec rec[SPLIT_SCENE_RECS];
-  const int C = SPLIT_SCENE_RECS / tg.ty;
+  const int C = SPLIT_SCENE_RECS / tg.ty;  // LDS: SPLIT_SCENE_RECS records (dynamic)
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= tg.ntiles()) return;  // whole wavefront
   const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
@@ -24,7 +23,7 @@ __global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, T
       f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
     } else {
       float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
-      split_span<ACCUM>(g, t, s, t.wlo, t.whi, rec, C, sr, si, cr, ci, ar, ai, cnt);
+      split_span<ACCUM>(g, t, s, t.wlo, t.whi, 0, C, sr, si, cr, ci, ar, ai, cnt);
       split_total<ACCUM>(sr, si, cr, ci, ar, ai);
     }
   }
@@ -39,10 +38,10 @@ int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Til
   const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
   if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
   if (t.accum == 0)
-    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), 0, st, ds->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, ds->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   else
-    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), 0, st, ds->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, ds->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "split kernel launch");

@@ -84,12 +84,20 @@ struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads
   float4 c;        // c1 = (k + 2 beta R0)/2pi, R0^2, 2R0, 1/(8R0^3)
 };
 
+// Dynamic LDS of every kernel that stages split records (indexed, so addressing stays 32-bit).
+extern __shared__ PulseRec sarbp_lds_rec[];
+
 __device__ __forceinline__ void make_rec(const BpGeom &g, double xc, double yc, double zc, double4 t,
                                          PulseRec &r) {
   const double dx = xc - t.x, dy = yc - t.y, dz = zc - t.z;
   const double r0sq = dx * dx + dy * dy + dz * dz;
-  const double r0 = __builtin_sqrt(r0sq);
-  const double inv = 1.0 / r0;
+  // 1/R0 from the hardware reciprocal square root and two Newton steps (a few ulp; R0 is only the
+  // expansion point, so it needs no correct rounding -- the pixel ranges are R0 + dR exactly)
+  double inv = __builtin_amdgcn_rsq(r0sq);
+  const double h = 0.5 * r0sq;
+  inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
+  inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
+  const double r0 = r0sq * inv;
   const double q0 = (r0 - g.near_range) * g.inv_dra;
   const double b0 = __builtin_rint(q0);
   const double ph0 = g.kphase * r0 + g.beta * r0sq;
@@ -151,25 +159,24 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 }
 
 // Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
-// and staged records do not depend on where a caller splits the range.  rec: this wavefront's
-// C * TY LDS records, record k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
+// and staged records do not depend on where a caller splits the range.  The wavefront's C * TY
+// LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
 // stages records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for
 // ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
-template <int ACCUM>
-__device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
-                                           int p1, PulseRec *rec, int C, float &sr, float &si, float &cr,
-                                           float &ci, double &ar, double &ai, int32_t &cnt) {
+template <int ACCUM, bool FOURTH>
+__device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
+                                             int p1, int rec0, int C, float &sr, float &si, float &cr,
+                                             float &ci, double &ar, double &ai, int32_t &cnt) {
+  PulseRec *const rec = sarbp_lds_rec;
   const int lane = threadIdx.x & 63;
-  const bool fourth = s.omax > SPLIT_FOURTH_M;
   const float inv_dra = (float)g.inv_dra;
   const float beta_t = (float)(g.beta * INV_2PI);
   const unsigned nbins = (unsigned)(g.nra - 1);
   const int nrec = C << s.ty_shift;
-  const PulseRec *my = rec + s.row;
   for (; c0 < p1; c0 += C) {
     for (int m = lane; m < nrec; m += 64) {
       const int p = c0 + (m >> s.ty_shift);
-      if (p < p1) make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[m]);
+      if (p < p1) make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + m]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -177,14 +184,14 @@ __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, c
     const int cn = min(C, p1 - c0);
     for (int k = 0; k < cn; ++k) {
       const int a = c0 + k;
-      const PulseRec *rk = my + (k << s.ty_shift);
-      const float4 ra = rk->a, rb = rk->b, rc = rk->c;
+      const int ri = rec0 + s.row + (k << s.ty_shift);
+      const float4 ra = rec[ri].a, rb = rec[ri].b, rc = rec[ri].c;
       if (a >= t.lo && a < t.hi) {
         const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
         const float t2 = fmaf(-av, av, s.qo);
         float dR = fmaf(t2, ra.w, av);
         dR = fmaf(-(av * t2), rb.x, dR);
-        if (fourth) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
+        if (FOURTH) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
         const float qb = fmaf(dR, inv_dra, rb.y);
         const float rq = __builtin_rintf(qb);
         int bin;
@@ -195,7 +202,8 @@ __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, c
           bin = __float_as_int(rb.z) + (int)rq;
         }
         if ((unsigned)bin < nbins) {
-          const float2 d = g.raw[(int64_t)a * g.nra + bin];
+          const float2 *col = g.raw + (int64_t)a * g.nra;  // wave-uniform column of pulse a
+          const float2 d = col[(unsigned)bin];
           const float ph = rc.x * dR;
           const float phl = fmaf(rc.x, dR, -ph);  // exact residual of the product
           const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, rb.w) + phl);
@@ -230,6 +238,17 @@ __device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, c
     __builtin_amdgcn_wave_barrier();  // reads of this chunk finish before the next overwrites rec
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+}
+
+// rec0: first LDS record (of sarbp_lds_rec) owned by this wavefront; it owns C * TY records.
+template <int ACCUM>
+__device__ __forceinline__ void split_span(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0, int p1,
+                                           int rec0, int C, float &sr, float &si, float &cr, float &ci,
+                                           double &ar, double &ai, int32_t &cnt) {
+  if (s.omax > SPLIT_FOURTH_M)
+    split_span_t<ACCUM, true>(g, t, s, c0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
+  else
+    split_span_t<ACCUM, false>(g, t, s, c0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
 }
 
 // Final sums of the split form as FP64.
