@@ -252,7 +252,8 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
       return rc;
     }
     b->total_px += reqs[r].n;
-    b->max_rows = std::max(b->max_rows, b->reqs[(size_t)r].tg.ty);
+    if (reqs[r].precision == SARBP_PREC_SPLIT)  // only split tiles stage LDS records
+      b->max_rows = std::max(b->max_rows, b->reqs[(size_t)r].tg.ty);
   }
   hipError_t e = hipStreamSynchronize(ds->stream);
   if (e != hipSuccess) {
