@@ -174,6 +174,17 @@ def main():
     r = res[head]
     kernel_s = r["kernel_s"]
     achieved = BYTES_PER_PP * pp / kernel_s / 1e9
+    traffic, traffic_src = None, None
+    tf = os.path.join(ROOT, "profiles", "traffic_full-4000x250x40001.json")
+    kname = {"split": "k_bp_split_tile<0>", "f64": "k_bp_f64_tile"}[head]
+    default_map = (head == "split" and args.split_map == "32,0") or (head == "f64" and args.f64_map == "1,8")
+    if not args.small and default_map and not args.sweep and os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        if tj.get("pp_per_launch") == pp and kname in tj.get("kernels", {}):
+            traffic = int(tj["kernels"][kname]["traffic_bytes"])
+            traffic_src = ("profiles/traffic_full-4000x250x40001.json: rocprofv3 --pmc FETCH_SIZE x%.3f + WRITE_SIZE x%.3f "
+                           "per launch of %s (scales from a 1 GiB copy probe)" % (tj["fetch_scale"], tj["write_scale"], kname))
     line = {
         "metric": "pixel·pulses/s split-FP32 vs FP64 at ≤1e-3 rad phase err; p50/p99 request latency",
         "value": r["pp_per_s"],
@@ -191,9 +202,12 @@ def main():
                                f"Sint U[2500,3500], mask 0.95",
                    "precision": head, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
     }
+    if traffic_src:
+        extra["traffic_source"] = traffic_src
+        extra["traffic_over_algorithmic"] = traffic / (BYTES_PER_PP * pp)
     line.update(extra)
     scene.close()
     ds.close()
