@@ -151,6 +151,74 @@ int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, in
 int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t *ncontrib);
 int sarbp_batch_destroy(sarbp_batch *b);
 
+/* ---- serving: continuous batching of requests against resident datasets -------------------- */
+
+/*
+ * A server owns one scheduling thread.  Submitted requests wait in a FIFO until admitted; admission
+ * needs a free slot and device memory within hbm_budget_bytes (the request's pixel buffers,
+ * outputs and partial state; the datasets themselves are resident already).  Each scheduling round
+ * is ONE persistent launch over every admitted request, advancing each tile of each request by up to
+ * the round's pulse span; requests that arrive while a round runs join the next round.  The partial
+ * sums of every pixel stay on the device between rounds (the per-request checkpoint), so a request
+ * can be preempted (its state copied to the host and its device memory released) and resumed later
+ * with a bit-identical result; results never depend on which requests share a round.
+ * A split-precision request whose a-priori phase-error estimate exceeds phase_err_limit_rad is
+ * focused in FP64 instead.  No counterpart in the reference, whose drivers process one dataset
+ * pass at a time (main_gpu.py:415-575).
+ */
+typedef struct sarbp_server sarbp_server;
+
+typedef struct sarbp_server_config {
+  int64_t hbm_budget_bytes;   /* admission budget for request buffers (0: 200 GB) */
+  double quantum_ms;          /* target GPU time of one scheduling round (0: 20 ms) */
+  int32_t max_active;         /* admitted requests at once (0: 256) */
+  int32_t pulse_chunk;        /* pulse granularity of round spans and LDS refills (0: 64) */
+  double phase_err_limit_rad; /* split -> FP64 fallback threshold (0: 1e-3) */
+} sarbp_server_config;
+
+#define SARBP_REQ_QUEUED 0
+#define SARBP_REQ_RUNNING 1
+#define SARBP_REQ_DONE 2
+#define SARBP_REQ_PREEMPTED 3
+#define SARBP_REQ_FAILED 4
+
+typedef struct sarbp_request_info {
+  int32_t state;             /* SARBP_REQ_* */
+  int32_t precision;         /* precision actually used (after the FP64 fallback decision) */
+  double phase_err_est_rad;  /* a-priori split phase-error estimate */
+  double submit_s, admit_s, done_s; /* server clock (seconds since server creation) */
+  int64_t pp;                /* pixel-pulse pairs of the request */
+  int32_t rounds;            /* scheduling rounds the request took part in */
+  int32_t preemptions;
+} sarbp_request_info;
+
+typedef struct sarbp_server_stats {
+  int64_t submitted, completed, failed, rounds;
+  int64_t pp_done;            /* pixel-pulse pairs of completed requests */
+  double gpu_busy_s;          /* summed kernel time of all rounds */
+  int64_t device_bytes_in_use;
+  int32_t active, queued;
+} sarbp_server_stats;
+
+int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
+/* Copy the request's inputs and queue it.  out_c128 (2*n float64) and ncontrib (n int32,
+ * nullable) receive the result when the request completes and must stay valid until then.
+ * SARBP_EBUDGET if the request alone exceeds the budget. */
+int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
+                        int32_t *ncontrib, int64_t *id);
+int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state);
+/* Block until done (SARBP_OK), failed (SARBP_EDEVICE) or timeout_s elapsed (SARBP_EAGAIN). */
+int sarbp_server_wait(sarbp_server *s, int64_t id, double timeout_s);
+/* Stop scheduling the request at the next round boundary, save its partial state on the host and
+ * release its device memory; SARBP_ESTATE if it is already done. */
+int sarbp_server_preempt(sarbp_server *s, int64_t id);
+/* Queue a preempted request again; it continues from its saved state. */
+int sarbp_server_resume(sarbp_server *s, int64_t id);
+int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info);
+int sarbp_server_get_stats(sarbp_server *s, sarbp_server_stats *st);
+/* Finish the current round, fail the requests still queued or running, join the thread. */
+int sarbp_server_destroy(sarbp_server *s);
+
 /*
  * Pixel-pulse pairs evaluated by a focusing pass (host-side, integers only): pixels with
  * mask == 1 and sint != -9999, pulses 0 < index + j - ceil(|sint|/2) < naz - 1 for

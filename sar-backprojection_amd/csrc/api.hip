@@ -49,6 +49,9 @@ int check_device() {
 }
 
 int upload_traj(sarbp_dataset *ds, const double *tx, const double *ty, const double *tz) {
+  ds->htx.assign(tx, tx + ds->naz);
+  ds->hty.assign(ty, ty + ds->naz);
+  ds->htz.assign(tz, tz + ds->naz);
   std::vector<double4> t((size_t)ds->naz);
   for (int64_t a = 0; a < ds->naz; ++a) t[(size_t)a] = make_double4(tx[a], ty[a], tz[a], 0.0);
   return h2d(ds->traj, t.data(), t.size() * sizeof(double4), ds->stream, "trajectory upload");

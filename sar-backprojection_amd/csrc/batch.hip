@@ -74,6 +74,14 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
       return sarbp_hip_fail(e, "request upload");
     }
   }
+  if (n) {  // pixels of tiles that are never scheduled (no aperture) read as zero
+    hipError_t e = hipMemsetAsync(b.out, 0, n * 16, st);
+    if (e == hipSuccess) e = hipMemsetAsync(b.ncnt, 0, n * 4, st);
+    if (e != hipSuccess) {
+      reqbuf_free(b);
+      return sarbp_hip_fail(e, "request output reset");
+    }
+  }
   // per-tile union lengths (host, integers only): work estimate for scheduling
   const int64_t nt = b.tg.ntiles();
   b.tile_len.assign((size_t)nt, 0);
@@ -102,8 +110,9 @@ void reqbuf_free(ReqBuf &b) {
   b.base = nullptr;
 }
 
-BatchReqDev reqbuf_dev(const ReqBuf &b) {
+BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
   BatchReqDev d;
+  d.g = ds->geom();
   d.px = PixelsView{b.x, b.y, b.z, b.mask, b.index, b.sint, b.n};
   d.out = b.out;
   d.ncnt = b.ncnt;
@@ -187,7 +196,7 @@ int batch_build(sarbp_batch *b, int waves) {
   std::vector<WorkItemCost> work;
   for (size_t r = 0; r < b->reqs.size(); ++r) {
     const ReqBuf &q = b->reqs[r];
-    devs.push_back(reqbuf_dev(q));
+    devs.push_back(reqbuf_dev(b->ds, q));
     const int64_t nt = q.tg.ntiles();
     for (int64_t t = 0; t < nt; t += waves) {
       int64_t c = 0;
@@ -212,8 +221,8 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
   if (b->ib.nitems > 0) {
-    rc = sarbp_launch_batch(ds, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk, b->max_rows,
-                            0, ds->stream);
+    rc = sarbp_launch_batch(ds->device, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk,
+                            b->max_rows, 0, ds->stream);
     if (rc) return rc;
   }
   e = hipEventRecord(ds->ev1, ds->stream);

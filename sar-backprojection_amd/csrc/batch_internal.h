@@ -25,7 +25,7 @@ struct ReqBuf {
 // Allocate and upload one request (asynchronously on st; the caller synchronises).
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b, hipStream_t st);
 void reqbuf_free(ReqBuf &b);
-BatchReqDev reqbuf_dev(const ReqBuf &b);
+BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 
 struct WorkItemCost {
   BatchItem item;

@@ -17,8 +17,9 @@
 
 namespace {
 
-__device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r, const BatchItem &it,
-                                           int64_t tile, int rec0, int C) {
+__device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem &it, int64_t tile, int rec0,
+                                           int C) {
+  const BpGeom g = r.g;
   const TileLane t = tile_lane(g, r.px, r.tg, tile, threadIdx.x & 63);
   const bool first = it.p_off == 0;
   if (t.wlo >= t.whi) {  // nothing to focus in this tile
@@ -77,7 +78,7 @@ __device__ __forceinline__ void batch_tile(const BpGeom &g, const BatchReqDev &r
 }
 
 // C pulses of records per refill, up to max_rows rows per tile: C * max_rows records per wavefront.
-__global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *__restrict__ reqs,
+__global__ __launch_bounds__(256) void k_bp_batch(const BatchReqDev *__restrict__ reqs,
                                                   const BatchItem *__restrict__ items,
                                                   const int32_t *__restrict__ seg, int32_t *heads, int C,
                                                   int max_rows) {
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_bp_batch(BpGeom g, const BatchReqDev *_
     const BatchItem it = items[k];
     const BatchReqDev &r = reqs[it.req];
     const int64_t tile = (int64_t)it.tile_lo + wave;
-    if (tile < r.tg.ntiles()) batch_tile(g, r, it, tile, rec0, C);
+    if (tile < r.tg.ntiles()) batch_tile(r, it, tile, rec0, C);
     (void)waves;
   }
 }
@@ -110,9 +111,8 @@ size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
   return (size_t)waves * (size_t)chunk * (size_t)max_rows * sizeof(PulseRec);
 }
 
-int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
-                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks,
-                       hipStream_t st) {
+int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
+                       int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks, hipStream_t st) {
   if (waves != 1 && waves != 2 && waves != 4) return sarbp_fail(SARBP_EINVAL, "waves per block must be 1, 2 or 4");
   if (chunk < 8 || chunk > 1024 || chunk % 8) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 8..1024, x8");
   if (max_rows < 1 || max_rows > 64) return sarbp_fail(SARBP_EINVAL, "bad tile row count");
@@ -123,13 +123,13 @@ int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const
   if (grid_blocks <= 0) {
     int per_cu = 0, ncu = 0;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bp_batch, waves * 64, lds);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ds->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) return sarbp_hip_fail(e, "occupancy query");
     if (per_cu < 1) per_cu = 1;
     grid_blocks = per_cu * ncu;
   }
-  hipLaunchKernelGGL(k_bp_batch, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, ds->geom(), d_reqs, d_items,
-                     d_seg, d_heads, chunk, max_rows);
+  hipLaunchKernelGGL(k_bp_batch, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, d_reqs, d_items, d_seg,
+                     d_heads, chunk, max_rows);
   e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "batch kernel launch");
   return SARBP_OK;

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "sarbp.h"
 
@@ -46,6 +47,7 @@ struct sarbp_dataset {
   double near_range = 0, dra = 0, lambda_val = 0;
   float2 *raw = nullptr;
   double4 *traj = nullptr;
+  std::vector<double> htx, hty, htz;  // host copy of the trajectory (request planning)
   int64_t bytes = 0;
   BpGeom geom() const;
 };
@@ -167,8 +169,10 @@ struct BpState {
   int32_t cnt, pad;
 };
 
-// One request as the batch kernel sees it (device pointers).
+// One request as the batch kernel sees it (device pointers).  Each request carries its dataset's
+// geometry, so one launch can mix requests against different device-resident datasets.
 struct BatchReqDev {
+  BpGeom g;
   PixelsView px;
   double2 *out;
   int32_t *ncnt;
@@ -194,7 +198,6 @@ constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
 // tiles of at most max_rows rows (LDS holds C * max_rows pulse records per wavefront).
 // d_heads: SARBP_BATCH_QUEUES ints, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item
 // offsets.  grid_blocks <= 0 sizes the grid from the occupancy.
-int sarbp_launch_batch(const sarbp_dataset *ds, const BatchReqDev *d_reqs, const BatchItem *d_items,
-                       const int32_t *d_seg, int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks,
-                       hipStream_t st);
+int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
+                       int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks, hipStream_t st);
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);

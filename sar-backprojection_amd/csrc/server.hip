@@ -1,0 +1,493 @@
+// Serving loop: continuous batching of focusing requests against device-resident datasets.
+//
+// One scheduling thread per server.  Each round it admits queued requests (FIFO) while a slot and
+// device memory within the budget are free, then focuses every admitted request in ONE launch of
+// the persistent batch kernel (bp_batch.hip), each tile advanced by the round's pulse span.  Partial
+// per-pixel sums stay in device memory between rounds; completed requests are copied out and their
+// memory released, preempted ones have their partial state saved on the host.  Results do not
+// depend on the mix or on the round boundaries: the per-pixel arithmetic and its order are fixed by
+// the request alone.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "batch_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct ServerReq {
+  int64_t id = 0;
+  sarbp_dataset *ds = nullptr;
+  std::vector<double> x, y, z;
+  std::vector<uint8_t> mask;
+  std::vector<int32_t> index, sint;
+  sarbp_request req{};  // points into the host copies above
+  double *out = nullptr;
+  int32_t *ncnt = nullptr;
+  int32_t state = SARBP_REQ_QUEUED;
+  int32_t precision = SARBP_PREC_F64;
+  double est = 0.0;
+  ReqBuf buf;
+  bool on_device = false;
+  int off = 0;       // pulses of every tile's union already focused (multiple of the chunk)
+  int max_len = 0;   // longest tile union (pulses)
+  int64_t need = 0;  // device bytes while admitted
+  std::vector<BpState> saved;  // partial state of a preempted request
+  bool preempt = false;
+  double submit_s = 0, admit_s = -1, done_s = -1;
+  int32_t rounds = 0, preemptions = 0;
+  int64_t pp = 0;
+};
+
+// A-priori phase error (rad) of the split form for a request: the FP32 error of the range offset
+// dR ~ 1.2e-7 x |a|, with |a| = |u.o| bounded per tile row from the row's offsets and the largest
+// along-track direction cosine over the pixels' apertures; plus ~2e-6 rad for the base phase and
+// the hardware sin/cos.
+double split_phase_estimate(const sarbp_dataset *ds, const sarbp_request &r, int tile_x, const double *tx,
+                            const double *ty, const double *tz) {
+  if (r.n == 0) return 0.0;
+  const int64_t dimx = r.dimx > 0 ? r.dimx : r.n, dimy = r.n / dimx;
+  const double kabs = std::fabs(-4.0 * SARBP_PI32_F64 / ds->lambda_val);
+  double amax = 0.0;
+  for (int64_t iy = 0; iy < dimy; ++iy)
+    for (int64_t x0 = 0; x0 < dimx; x0 += tile_x) {
+      const int64_t x1 = std::min(dimx, x0 + tile_x);
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, cmax = 0.0;
+      bool any = false;
+      for (int64_t ix = x0; ix < x1; ++ix) {
+        const int64_t i = ix + dimx * iy;
+        if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
+        int64_t a0, a1;
+        sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, a0, a1);
+        if (a1 <= a0) continue;
+        any = true;
+        const double p[3] = {r.x[i], r.y[i], r.z[i]};
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], p[k]);
+          hi[k] = std::max(hi[k], p[k]);
+        }
+        for (int64_t a : {a0, a1 - 1}) {  // direction cosine along x at the aperture ends
+          const double dx = p[0] - tx[a], dy = p[1] - ty[a], dz = p[2] - tz[a];
+          cmax = std::max(cmax, std::fabs(dx) / std::sqrt(dx * dx + dy * dy + dz * dz));
+        }
+      }
+      if (!any) continue;
+      const double ox = 0.5 * (hi[0] - lo[0]), oy = 0.5 * (hi[1] - lo[1]), oz = 0.5 * (hi[2] - lo[2]);
+      amax = std::max(amax, ox * cmax + oy + oz);
+      if (std::sqrt(ox * ox + oy * oy + oz * oz) > 48.0) return 1e9;  // kernel takes FP64 terms anyway
+    }
+  return kabs * 1.2e-7 * amax + 2e-6;
+}
+
+}  // namespace
+
+struct sarbp_server {
+  sarbp_server_config cfg{};
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::map<int64_t, std::unique_ptr<ServerReq>> reqs;
+  std::deque<ServerReq *> queue;
+  std::vector<ServerReq *> active;
+  int64_t next_id = 1;
+  bool stop = false;
+  bool started = false;
+  int device = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ItemBuffers ib;
+  int64_t bytes_in_use = 0;
+  double rate = 3e11;  // pixel-pulse pairs per second, running estimate
+  Clock::time_point t0 = Clock::now();
+  sarbp_server_stats stats{};
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+  void run();
+  bool admit(ServerReq *r);
+  void release(ServerReq *r) {
+    reqbuf_free(r->buf);
+    r->on_device = false;
+    bytes_in_use -= r->need;
+  }
+};
+
+bool sarbp_server::admit(ServerReq *r) {
+  int rc = reqbuf_create(r->ds, r->req, true, default_tiling(r->precision).accum, r->buf, st);
+  if (rc) return false;
+  if (!r->saved.empty()) {
+    hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), r->saved.size() * sizeof(BpState),
+                                  hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) {
+      reqbuf_free(r->buf);
+      return false;
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    reqbuf_free(r->buf);
+    return false;
+  }
+  r->saved.clear();
+  r->saved.shrink_to_fit();
+  r->on_device = true;
+  r->max_len = 0;
+  for (int v : r->buf.tile_len) r->max_len = std::max(r->max_len, v);
+  return true;
+}
+
+void sarbp_server::run() {
+  (void)hipSetDevice(device);
+  const int chunk = cfg.pulse_chunk;
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    // admission, FIFO: a request waits until a slot and its memory are free
+    while (!stop && !queue.empty() && (int)active.size() < cfg.max_active) {
+      ServerReq *r = queue.front();
+      if (bytes_in_use + r->need > cfg.hbm_budget_bytes && !active.empty()) break;
+      queue.pop_front();
+      bytes_in_use += r->need;
+      r->state = SARBP_REQ_RUNNING;
+      lk.unlock();
+      const bool ok = admit(r);
+      lk.lock();
+      if (!ok) {
+        bytes_in_use -= r->need;
+        r->state = SARBP_REQ_FAILED;
+        stats.failed++;
+        cv_done.notify_all();
+        continue;
+      }
+      if (r->admit_s < 0) r->admit_s = now();
+      active.push_back(r);
+    }
+    if (stop) break;
+    // requests asked to stop before any work this round
+    for (auto it = active.begin(); it != active.end();) {
+      ServerReq *r = *it;
+      if (!r->preempt) {
+        ++it;
+        continue;
+      }
+      r->saved.resize((size_t)r->buf.n);
+      lk.unlock();
+      hipError_t e = r->buf.n ? hipMemcpyAsync(r->saved.data(), r->buf.state, r->saved.size() * sizeof(BpState),
+                                               hipMemcpyDeviceToHost, st)
+                              : hipSuccess;
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      lk.lock();
+      release(r);
+      r->preempt = false;
+      r->state = e == hipSuccess ? SARBP_REQ_PREEMPTED : SARBP_REQ_FAILED;
+      if (e != hipSuccess) stats.failed++;
+      r->preemptions++;
+      it = active.erase(it);
+      cv_done.notify_all();
+    }
+    if (active.empty()) {
+      cv_work.wait_for(lk, std::chrono::milliseconds(20));
+      continue;
+    }
+    // round: every active request advances every unfinished tile by `span` pulses
+    int64_t live_px = 0;
+    for (ServerReq *r : active) live_px += std::max<int64_t>(1, r->buf.n);
+    const double budget_pp = rate * cfg.quantum_ms * 1e-3;
+    int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)live_px));
+    span = std::max(chunk, (span + chunk - 1) / chunk * chunk);
+    std::vector<ServerReq *> round(active);
+    std::vector<BatchReqDev> devs;
+    std::vector<WorkItemCost> work;
+    int max_rows = 1;
+    int64_t round_pp = 0;
+    for (size_t k = 0; k < round.size(); ++k) {
+      ServerReq *r = round[k];
+      devs.push_back(reqbuf_dev(r->ds, r->buf));
+      if (r->precision == SARBP_PREC_SPLIT) max_rows = std::max(max_rows, r->buf.tg.ty);
+      const int64_t nt = r->buf.tg.ntiles();
+      const int p_end = r->off + span;
+      for (int64_t t = 0; t < nt; ++t) {
+        const int len = r->buf.tile_len[(size_t)t];
+        if (len <= r->off) continue;  // finished (or empty: its zeros are written at admission)
+        const int c = std::min(len, p_end) - r->off;
+        work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, r->off, p_end}, c});
+        round_pp += (int64_t)c * 64;
+      }
+    }
+    std::vector<BatchItem> items;
+    std::vector<int32_t> seg;
+    build_queues(work, items, seg);
+    lk.unlock();
+    float ms = 0.f;
+    int rc = ib.upload(devs, items, seg, st);
+    hipError_t e = hipSuccess;
+    if (!rc && !items.empty()) {
+      e = hipEventRecord(e0, st);
+      if (e == hipSuccess)
+        rc = sarbp_launch_batch(device, ib.d_reqs, ib.d_items, ib.d_seg, ib.d_heads, 1, chunk, max_rows, 0, st);
+      if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
+      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    lk.lock();
+    stats.rounds++;
+    stats.gpu_busy_s += ms * 1e-3;
+    if (ms > 0.05f && round_pp > 0) rate = 0.7 * rate + 0.3 * (round_pp * 0.8 / (ms * 1e-3));  // lanes ~80 % active
+    const bool failed = rc || e != hipSuccess;
+    for (ServerReq *r : round) {
+      r->rounds++;
+      if (failed) {
+        release(r);
+        r->state = SARBP_REQ_FAILED;
+        stats.failed++;
+        continue;
+      }
+      r->off += span;
+      if (r->off >= r->max_len) {  // every tile wrote its means: copy the result out
+        lk.unlock();
+        hipError_t ce = hipSuccess;
+        if (r->out && r->buf.n)
+          ce = hipMemcpyAsync(r->out, r->buf.out, (size_t)r->buf.n * sizeof(double2), hipMemcpyDeviceToHost, st);
+        if (ce == hipSuccess && r->ncnt && r->buf.n)
+          ce = hipMemcpyAsync(r->ncnt, r->buf.ncnt, (size_t)r->buf.n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        if (ce == hipSuccess) ce = hipStreamSynchronize(st);
+        lk.lock();
+        release(r);
+        r->done_s = now();
+        if (ce == hipSuccess) {
+          r->state = SARBP_REQ_DONE;
+          stats.completed++;
+          stats.pp_done += r->pp;
+        } else {
+          r->state = SARBP_REQ_FAILED;
+          stats.failed++;
+        }
+      }
+    }
+    active.erase(std::remove_if(active.begin(), active.end(),
+                                [](ServerReq *r) { return r->state != SARBP_REQ_RUNNING; }),
+                 active.end());
+    cv_done.notify_all();
+  }
+  // shutdown: fail what is left
+  for (ServerReq *r : active) {
+    release(r);
+    r->state = SARBP_REQ_FAILED;
+  }
+  active.clear();
+  for (ServerReq *r : queue) r->state = SARBP_REQ_FAILED;
+  queue.clear();
+  cv_done.notify_all();
+}
+
+extern "C" {
+
+int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
+  if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
+  *out = nullptr;
+  sarbp_server *s = new (std::nothrow) sarbp_server();
+  if (!s) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
+  sarbp_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)200e9;
+  if (c.quantum_ms <= 0) c.quantum_ms = 20.0;
+  if (c.max_active <= 0) c.max_active = 256;
+  if (c.pulse_chunk <= 0) c.pulse_chunk = 64;
+  if (c.phase_err_limit_rad <= 0) c.phase_err_limit_rad = 1e-3;
+  if (c.pulse_chunk % 8 || c.pulse_chunk > 1024) {
+    delete s;
+    return sarbp_fail(SARBP_EINVAL, "pulse_chunk must be a multiple of 8, at most 1024");
+  }
+  s->cfg = c;
+  hipError_t e = hipGetDevice(&s->device);
+  if (e == hipSuccess) e = hipStreamCreate(&s->st);
+  if (e == hipSuccess) e = hipEventCreate(&s->e0);
+  if (e == hipSuccess) e = hipEventCreate(&s->e1);
+  if (e != hipSuccess) {
+    int rc = sarbp_hip_fail(e, "server stream");
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+    return rc;
+  }
+  try {
+    s->th = std::thread([s] { s->run(); });
+  } catch (...) {
+    (void)hipStreamDestroy(s->st);
+    delete s;
+    return sarbp_fail(SARBP_ENOMEM, "could not start the scheduling thread");
+  }
+  s->started = true;
+  *out = s;
+  return SARBP_OK;
+}
+
+int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
+                        int32_t *ncontrib, int64_t *id) {
+  if (!s || !ds || !req || !id) return sarbp_fail(SARBP_EINVAL, "null argument");
+  if (req->n < 0 || (req->n > 0 && (!req->x || !req->y || !req->z || !req->mask || !req->index || !req->sint)))
+    return sarbp_fail(SARBP_EINVAL, "bad request pixel arrays");
+  if (req->n > 0 && !out_c128) return sarbp_fail(SARBP_EINVAL, "output buffer is null");
+  if (req->dimx < 0 || (req->n > 0 && req->dimx > 0 && req->n % req->dimx))
+    return sarbp_fail(SARBP_EINVAL, "request pixel count must be a multiple of dimx");
+  if (req->precision != SARBP_PREC_F64 && req->precision != SARBP_PREC_SPLIT)
+    return sarbp_fail(SARBP_EINVAL, "unknown request precision");
+  auto r = std::make_unique<ServerReq>();
+  const size_t n = (size_t)req->n;
+  r->x.assign(req->x, req->x + n);
+  r->y.assign(req->y, req->y + n);
+  r->z.assign(req->z, req->z + n);
+  r->mask.assign(req->mask, req->mask + n);
+  r->index.assign(req->index, req->index + n);
+  r->sint.assign(req->sint, req->sint + n);
+  r->req = *req;
+  r->req.x = r->x.data();
+  r->req.y = r->y.data();
+  r->req.z = r->z.data();
+  r->req.mask = r->mask.data();
+  r->req.index = r->index.data();
+  r->req.sint = r->sint.data();
+  r->ds = ds;
+  r->out = out_c128;
+  r->ncnt = ncontrib;
+  const int tx = req->tile_x ? req->tile_x : default_tiling(req->precision).tile_x;
+  if (tx != 64 && tx != 32 && tx != 16 && tx != 8 && tx != 4) return sarbp_fail(SARBP_EINVAL, "bad tile_x");
+  r->precision = req->precision;
+  if (req->precision == SARBP_PREC_SPLIT) {
+    r->est = split_phase_estimate(ds, r->req, tx, ds->htx.data(), ds->hty.data(), ds->htz.data());
+    if (r->est > s->cfg.phase_err_limit_rad) r->precision = SARBP_PREC_F64;
+  }
+  r->req.precision = r->precision;
+  r->req.tile_x = r->precision == req->precision ? tx : 0;
+  // device bytes while admitted: pixels + outputs + state (each array 256-byte aligned)
+  auto up = [](int64_t v) { return (v + 255) / 256 * 256; };
+  const int64_t nn = req->n;
+  r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) + up(nn * (int64_t)sizeof(BpState));
+  if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
+  for (size_t i = 0; i < n; ++i) {
+    if (r->mask[i] != 1 || r->sint[i] == -9999) continue;
+    int64_t lo, hi;
+    sarbp_pulse_range(r->index[i], r->sint[i], ds->naz, lo, hi);
+    r->pp += hi - lo;
+  }
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return sarbp_fail(SARBP_ESTATE, "server is shutting down");
+  r->id = s->next_id++;
+  r->submit_s = s->now();
+  *id = r->id;
+  s->queue.push_back(r.get());
+  s->reqs[r->id] = std::move(r);
+  s->stats.submitted++;
+  s->cv_work.notify_all();
+  return SARBP_OK;
+}
+
+int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state) {
+  if (!s || !state) return sarbp_fail(SARBP_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  *state = it->second->state;
+  return SARBP_OK;
+}
+
+int sarbp_server_wait(sarbp_server *s, int64_t id, double timeout_s) {
+  if (!s) return sarbp_fail(SARBP_EINVAL, "null handle");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  ServerReq *r = it->second.get();
+  auto fin = [r] { return r->state == SARBP_REQ_DONE || r->state == SARBP_REQ_FAILED || r->state == SARBP_REQ_PREEMPTED; };
+  if (timeout_s < 0) {
+    s->cv_done.wait(lk, fin);
+  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), fin)) {
+    return SARBP_EAGAIN;
+  }
+  if (r->state == SARBP_REQ_FAILED) return sarbp_fail(SARBP_EDEVICE, "request failed");
+  if (r->state == SARBP_REQ_PREEMPTED) return sarbp_fail(SARBP_ESTATE, "request is preempted");
+  return SARBP_OK;
+}
+
+int sarbp_server_preempt(sarbp_server *s, int64_t id) {
+  if (!s) return sarbp_fail(SARBP_EINVAL, "null handle");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  ServerReq *r = it->second.get();
+  if (r->state == SARBP_REQ_QUEUED) {
+    s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), r), s->queue.end());
+    r->state = SARBP_REQ_PREEMPTED;
+    r->preemptions++;
+    s->cv_done.notify_all();
+    return SARBP_OK;
+  }
+  if (r->state != SARBP_REQ_RUNNING) return sarbp_fail(SARBP_ESTATE, "request is not queued or running");
+  r->preempt = true;
+  s->cv_work.notify_all();
+  return SARBP_OK;
+}
+
+int sarbp_server_resume(sarbp_server *s, int64_t id) {
+  if (!s) return sarbp_fail(SARBP_EINVAL, "null handle");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  ServerReq *r = it->second.get();
+  if (r->state != SARBP_REQ_PREEMPTED) return sarbp_fail(SARBP_ESTATE, "request is not preempted");
+  r->state = SARBP_REQ_QUEUED;
+  s->queue.push_back(r);
+  s->cv_work.notify_all();
+  return SARBP_OK;
+}
+
+int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info) {
+  if (!s || !info) return sarbp_fail(SARBP_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  const ServerReq *r = it->second.get();
+  info->state = r->state;
+  info->precision = r->precision;
+  info->phase_err_est_rad = r->est;
+  info->submit_s = r->submit_s;
+  info->admit_s = r->admit_s;
+  info->done_s = r->done_s;
+  info->pp = r->pp;
+  info->rounds = r->rounds;
+  info->preemptions = r->preemptions;
+  return SARBP_OK;
+}
+
+int sarbp_server_get_stats(sarbp_server *s, sarbp_server_stats *st) {
+  if (!s || !st) return sarbp_fail(SARBP_EINVAL, "null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *st = s->stats;
+  st->device_bytes_in_use = s->bytes_in_use;
+  st->active = (int32_t)s->active.size();
+  st->queued = (int32_t)s->queue.size();
+  return SARBP_OK;
+}
+
+int sarbp_server_destroy(sarbp_server *s) {
+  if (!s) return SARBP_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  if (s->started && s->th.joinable()) s->th.join();
+  s->ib.release();
+  if (s->e0) (void)hipEventDestroy(s->e0);
+  if (s->e1) (void)hipEventDestroy(s->e1);
+  if (s->st) (void)hipStreamDestroy(s->st);
+  delete s;
+  return SARBP_OK;
+}
+
+}  // extern "C"

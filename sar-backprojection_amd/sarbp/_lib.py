@@ -40,6 +40,27 @@ class Request(ctypes.Structure):
 
 
 _RP = ctypes.POINTER(Request)
+
+
+class ServerConfig(ctypes.Structure):
+    """sarbp_server_config (include/sarbp.h); zero fields take the library defaults."""
+    _fields_ = [("hbm_budget_bytes", _i64), ("quantum_ms", _d), ("max_active", _i32), ("pulse_chunk", _i32),
+                ("phase_err_limit_rad", _d)]
+
+
+class RequestInfo(ctypes.Structure):
+    """sarbp_request_info (include/sarbp.h)."""
+    _fields_ = [("state", _i32), ("precision", _i32), ("phase_err_est_rad", _d), ("submit_s", _d),
+                ("admit_s", _d), ("done_s", _d), ("pp", _i64), ("rounds", _i32), ("preemptions", _i32)]
+
+
+class ServerStats(ctypes.Structure):
+    """sarbp_server_stats (include/sarbp.h)."""
+    _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("rounds", _i64), ("pp_done", _i64),
+                ("gpu_busy_s", _d), ("device_bytes_in_use", _i64), ("active", _i32), ("queued", _i32)]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_PREEMPTED, REQ_FAILED = 0, 1, 2, 3, 4
 _i32p = ctypes.POINTER(_i32)
 _fp = ctypes.POINTER(ctypes.c_float)
 
@@ -65,6 +86,15 @@ SIGNATURES = {
     "sarbp_batch_autotune": (ctypes.c_int, [_p, _i32, _i32p, _i32p, _fp]),
     "sarbp_batch_download": (ctypes.c_int, [_p, _i32, _p, _p]),
     "sarbp_batch_destroy": (ctypes.c_int, [_p]),
+    "sarbp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), _pp]),
+    "sarbp_server_submit": (ctypes.c_int, [_p, _p, _RP, _p, _p, ctypes.POINTER(_i64)]),
+    "sarbp_server_poll": (ctypes.c_int, [_p, _i64, _i32p]),
+    "sarbp_server_wait": (ctypes.c_int, [_p, _i64, _d]),
+    "sarbp_server_preempt": (ctypes.c_int, [_p, _i64]),
+    "sarbp_server_resume": (ctypes.c_int, [_p, _i64]),
+    "sarbp_server_info": (ctypes.c_int, [_p, _i64, ctypes.POINTER(RequestInfo)]),
+    "sarbp_server_get_stats": (ctypes.c_int, [_p, ctypes.POINTER(ServerStats)]),
+    "sarbp_server_destroy": (ctypes.c_int, [_p]),
 }
 
 _lib = None

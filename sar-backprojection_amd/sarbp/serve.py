@@ -1,0 +1,189 @@
+"""Serving: continuous batching of focusing requests on device-resident datasets (sarbp_server_*).
+
+:class:`Server` wraps the library's scheduling thread; :func:`poisson_run` drives it with Poisson
+arrivals of sub-scene requests and reports p50/p99 latency (completion - arrival, queueing
+included) and aggregate pixel-pulse pairs per second.
+
+    python -m sarbp.serve --rates 5,20,50 --duration 60
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import sys
+import threading
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import EAGAIN, check
+from .batch import Request
+
+
+class Handle:
+    """A submitted request: keeps the result buffers alive until completion."""
+
+    def __init__(self, server, rid, req: Request, out, cnt, t_submit):
+        self.server, self.id, self.req, self.out, self.cnt, self.t_submit = server, rid, req, out, cnt, t_submit
+
+    def wait(self, timeout=None) -> bool:
+        st = _lib.load().sarbp_server_wait(self.server._h, self.id, -1.0 if timeout is None else float(timeout))
+        if st == EAGAIN:
+            return False
+        check(st, "sarbp_server_wait")
+        return True
+
+    def state(self) -> int:
+        v = ctypes.c_int32()
+        check(_lib.load().sarbp_server_poll(self.server._h, self.id, ctypes.byref(v)), "sarbp_server_poll")
+        return int(v.value)
+
+    def info(self) -> dict:
+        i = _lib.RequestInfo()
+        check(_lib.load().sarbp_server_info(self.server._h, self.id, ctypes.byref(i)), "sarbp_server_info")
+        return {k: getattr(i, k) for k, _ in _lib.RequestInfo._fields_}
+
+    def preempt(self):
+        check(_lib.load().sarbp_server_preempt(self.server._h, self.id), "sarbp_server_preempt")
+
+    def resume(self):
+        check(_lib.load().sarbp_server_resume(self.server._h, self.id), "sarbp_server_resume")
+
+    def result(self, with_counts=False):
+        return (self.out, self.cnt) if with_counts else self.out
+
+
+class Server:
+    def __init__(self, hbm_budget_bytes=0, quantum_ms=0.0, max_active=0, pulse_chunk=0, phase_err_limit_rad=0.0):
+        _lib.require_device()
+        cfg = _lib.ServerConfig(int(hbm_budget_bytes), float(quantum_ms), int(max_active), int(pulse_chunk),
+                                float(phase_err_limit_rad))
+        h = ctypes.c_void_p()
+        check(_lib.load().sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)), "sarbp_server_create")
+        self._h = h
+
+    def submit(self, ds, req: Request) -> Handle:
+        out = np.zeros(req.n, np.complex128)
+        cnt = np.zeros(req.n, np.int32)
+        rid = ctypes.c_int64()
+        st = req.c_struct()
+        t = time.perf_counter()
+        check(_lib.load().sarbp_server_submit(self._h, ds.handle, ctypes.byref(st), out.ctypes.data, cnt.ctypes.data,
+                                              ctypes.byref(rid)), "sarbp_server_submit")
+        return Handle(self, int(rid.value), req, out, cnt, t)
+
+    def stats(self) -> dict:
+        s = _lib.ServerStats()
+        check(_lib.load().sarbp_server_get_stats(self._h, ctypes.byref(s)), "sarbp_server_get_stats")
+        return {k: getattr(s, k) for k, _ in _lib.ServerStats._fields_}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().sarbp_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v, np.float64), q)) if len(v) else None
+
+
+def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: float, seed: int = 0,
+                warmup_s: float = 2.0, log=None):
+    """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at Poisson times of
+    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival (server clock)."""
+    rng = np.random.default_rng(seed)
+    handles = []
+    t0 = time.perf_counter()
+    t_next = 0.0
+    k = 0
+    last_log = t0
+    while t_next < duration_s:
+        now = time.perf_counter() - t0
+        if t_next > now:
+            time.sleep(min(t_next - now, 0.05))
+            continue
+        di, req = pool[k % len(pool)]
+        handles.append((t_next, server.submit(datasets[di], req)))
+        k += 1
+        t_next += rng.exponential(1.0 / rate_hz)
+        if log and time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"rate {rate_hz}/s: {k} submitted, t={now:.0f}s, {server.stats()}")
+    for _, h in handles:
+        h.wait()
+    t_end = time.perf_counter() - t0
+    lat, pp, prec = [], 0, {"f64": 0, "split": 0}
+    first_submit, last_done = float("inf"), 0.0
+    for t_arr, h in handles:
+        inf = h.info()
+        if t_arr >= warmup_s:
+            lat.append(inf["done_s"] - inf["submit_s"])
+        pp += inf["pp"]
+        first_submit = min(first_submit, inf["submit_s"])
+        last_done = max(last_done, inf["done_s"])
+        prec["split" if inf["precision"] == _lib.PREC_SPLIT else "f64"] += 1
+    span = max(1e-9, last_done - first_submit)
+    return {
+        "rate_req_per_s": rate_hz, "duration_s": duration_s, "requests": len(handles),
+        "latency_p50_ms": percentile(lat, 50) * 1e3 if lat else None,
+        "latency_p99_ms": percentile(lat, 99) * 1e3 if lat else None,
+        "latency_max_ms": max(lat) * 1e3 if lat else None,
+        "pp_total": int(pp), "pp_per_s": pp / span, "wall_s": t_end, "precision_counts": prec,
+        "server": server.stats(),
+    }
+
+
+def make_pool(scenes, nreq_per_dataset=32, seed=0):
+    """Sub-scene requests (width 128-1024, height 32-250, Sint 500-4000, split:FP64 = 1:1)."""
+    from .batch import patch_requests
+    pool = []
+    for di, sc in enumerate(scenes):
+        for r in patch_requests(sc, nreq_per_dataset, seed=seed + di):
+            pool.append((di, r))
+    rng = np.random.default_rng(seed)
+    rng.shuffle(pool)
+    return pool
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="Poisson serving benchmark")
+    ap.add_argument("--rates", default="5,20,50")
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--datasets", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--quantum-ms", type=float, default=20.0)
+    ap.add_argument("--small", action="store_true")
+    a = ap.parse_args(argv)
+    from . import synth
+    from .core import Dataset
+    log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    base = synth.full_scene(seed=1, **({"nx": 512, "ny": 250, "naz": 5121, "ntarget": 200} if a.small else {}))
+    log(f"scene {base.name} ready")
+    datasets, scenes = [], []
+    for k in range(a.datasets):
+        raw = base.raw if k == 0 else np.asfortranarray(base.raw * np.complex64(np.exp(1j * 0.7 * k)))
+        datasets.append(Dataset(raw, base.tx, base.ty, base.tz, base.near, base.dra, base.lam))
+        scenes.append(base)
+        log(f"dataset {k} resident")
+    pool = make_pool(scenes)
+    srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms)
+    results = []
+    for rate in [float(r) for r in a.rates.split(",") if r]:
+        res = poisson_run(srv, datasets, pool, rate, a.duration, log=log)
+        log(json.dumps(res))
+        results.append(res)
+    srv.close()
+    print(json.dumps({"serving": results}))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,118 @@
+"""Serving loop: continuous batching, preemption/resume, admission budget, FP64 fallback (GPU).
+
+The bar for continuous batching: every request's result is bit-identical whether it ran alone or
+in a mix with others, and equal to the same request focused by the batch kernel alone.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from sarbp import _lib, synth
+from sarbp.batch import Batch, Request, patch_requests
+from sarbp.core import Dataset
+from sarbp.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def world():
+    sc = synth.full_scene(seed=7, nx=512, ny=250, naz=5121, nra=4762, ntarget=100)
+    ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    ds2 = Dataset(np.asfortranarray(sc.raw * np.complex64(1j)), sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    yield sc, ds, ds2
+    ds.close()
+    ds2.close()
+
+
+def test_alone_equals_mix_equals_batch(gpu, world):
+    sc, ds, ds2 = world
+    reqs = patch_requests(sc, 8, seed=3, w_range=(16, 80), h_range=(8, 40), sint_range=(500, 4000))
+    dsets = [ds if k % 3 else ds2 for k in range(len(reqs))]
+    print("case serve mix of", len(reqs), flush=True)
+    srv = Server(quantum_ms=2.0)
+    alone = []
+    for d, r in zip(dsets, reqs):
+        h = srv.submit(d, r)
+        assert h.wait(timeout=60)
+        alone.append(h.result(with_counts=True))
+    hs = [srv.submit(d, r) for d, r in zip(dsets, reqs)]
+    for h in hs:
+        assert h.wait(timeout=60)
+    rounds = [h.info()["rounds"] for h in hs]
+    for (a, ac), h in zip(alone, hs):
+        b, bc = h.result(with_counts=True)
+        assert np.array_equal(ac, bc) and np.array_equal(a, b)
+    # the one-shot batch kernel on the same requests gives the same bits
+    for d in (ds, ds2):
+        idx = [k for k in range(len(reqs)) if dsets[k] is d]
+        b = Batch(d, [reqs[k] for k in idx])
+        b.run(64, 64)
+        for j, k in enumerate(idx):
+            assert np.array_equal(b.result(j), alone[k][0])
+        b.close()
+    st = srv.stats()
+    assert st["completed"] == 2 * len(reqs) and st["failed"] == 0
+    print("rounds per request in the mix", rounds, st, flush=True)
+    srv.close()
+
+
+def test_preempt_and_resume_is_bit_identical(gpu, world):
+    sc, ds, _ = world
+    (req,) = patch_requests(sc, 1, seed=5, w_range=(200, 200), h_range=(64, 64), sint_range=(3800, 3800),
+                            precisions=("split",))
+    srv = Server(quantum_ms=0.05, pulse_chunk=64)  # ~64 pulses per round: many rounds
+    ref = srv.submit(ds, req)
+    assert ref.wait(timeout=120)
+    h = srv.submit(ds, req)
+    t0 = time.time()
+    while h.info()["rounds"] < 3 and time.time() - t0 < 60:
+        time.sleep(0.001)
+    h.preempt()
+    h.wait(timeout=60)
+    inf = h.info()
+    print("case preempt", inf, flush=True)
+    assert inf["state"] == _lib.REQ_PREEMPTED and inf["preemptions"] == 1
+    h.resume()
+    assert h.wait(timeout=120)
+    assert np.array_equal(h.result(), ref.result())
+    assert h.info()["rounds"] > inf["rounds"]
+    srv.close()
+
+
+def test_budget_admission_and_rejection(gpu, world):
+    sc, ds, _ = world
+    reqs = patch_requests(sc, 4, seed=8, w_range=(64, 64), h_range=(32, 32), sint_range=(1000, 1000))
+    per_req = 64 * 32 * (8 * 3 + 1 + 4 * 2 + 16 + 4 + 32) + 8 * 256
+    srv = Server(hbm_budget_bytes=int(per_req * 1.5), quantum_ms=1.0)  # room for one request at a time
+    hs = [srv.submit(ds, r) for r in reqs]
+    for h in hs:
+        assert h.wait(timeout=120)
+    admits = sorted((h.info()["admit_s"], h.info()["done_s"]) for h in hs)
+    for (a0, d0), (a1, d1) in zip(admits, admits[1:]):
+        assert a1 >= d0 - 1e-6  # the next request is admitted only after the previous one left
+    big = Request(*(np.tile(a, 4) for a in (reqs[0].x, reqs[0].y, reqs[0].z, reqs[0].mask, reqs[0].index,
+                                           reqs[0].sint)), dimx=64)
+    with pytest.raises(_lib.SarbpError):
+        srv.submit(ds, big)
+    srv.close()
+
+
+def test_split_falls_back_to_fp64_when_estimate_exceeds_limit(gpu, world):
+    sc, ds, _ = world
+    # a coarse 20 m grid: tile rows span hundreds of metres, the split estimate is far above 1e-3 rad
+    sub = sc.sub(0, 512, 0, 250)
+    pick = (slice(0, 512, 20), slice(0, 250, 20))
+    f = lambda a: np.asfortranarray(a[pick]).flatten(order="F")  # noqa: E731
+    nx = len(range(0, 512, 20))
+    req = Request(f(sub.x), f(sub.y), f(sub.z), f(sub.mask), f(sub.index), f(sub.sint), dimx=nx, precision="split")
+    fine = patch_requests(sc, 1, seed=9, w_range=(64, 64), h_range=(16, 16), precisions=("split",))[0]
+    srv = Server()
+    h1, h2 = srv.submit(ds, req), srv.submit(ds, fine)
+    assert h1.wait(60) and h2.wait(60)
+    i1, i2 = h1.info(), h2.info()
+    print("estimates", i1["phase_err_est_rad"], i2["phase_err_est_rad"], flush=True)
+    assert i1["precision"] == _lib.PREC_F64 and i1["phase_err_est_rad"] > 1e-3
+    assert i2["precision"] == _lib.PREC_SPLIT and i2["phase_err_est_rad"] < 1e-3
+    srv.close()
