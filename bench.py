@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
     ap.add_argument("--split-map", default="32,0", help="split work mapping tile_x,accum")
     ap.add_argument("--sweep", action="store_true", help="time every work mapping first and use the fastest")
+    ap.add_argument("--batch-requests", type=int, default=64, help="sub-scene requests of the batch leg (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson serving leg duration (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pixel sharding across GPUs "
@@ -156,6 +159,34 @@ def main():
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
         extra["split_vs_f64_errors"] = phase_amp_errors(outs["split"], outs["f64"])
+
+    # batch leg: uneven sub-scene requests in one persistent launch, mapping chosen by the autotuner
+    if args.batch_requests > 0:
+        from sarbp.batch import Batch, patch_requests
+        reqs = patch_requests(sc, args.batch_requests, seed=0)
+        b = Batch(ds, reqs)
+        bpp = b.count_pp()
+        tp, ch, tms = b.autotune(reps=1)
+        times = [b.run() for _ in range(max(1, args.steps))]
+        bms = float(np.mean(times))
+        extra["batch"] = {"requests": len(reqs), "pixels": int(sum(r.n for r in reqs)), "pp": int(bpp),
+                          "ms_per_launch": bms, "pp_per_s": bpp / (bms * 1e-3), "tile_px": tp, "pulse_chunk": ch,
+                          "precisions": "split:f64 = 1:1, widths 128-1024, heights 32-250, Sint 500-4000"}
+        log(f"batch of {len(reqs)}: {bms:.2f} ms, {bpp / (bms * 1e-3):.4e} pp/s (tile_px {tp}, chunk {ch})")
+        b.close()
+    # serving leg: Poisson arrivals of the same kind of requests, latency = completion - arrival
+    if args.serve_seconds > 0:
+        from sarbp.serve import Server, make_pool, poisson_run
+        srv = Server()
+        pool = make_pool([sc], nreq_per_dataset=32, seed=1)
+        sv = poisson_run(srv, [ds], pool, args.serve_rate, args.serve_seconds, warmup_s=1.0)
+        srv.close()
+        extra["serving"] = {k: sv[k] for k in ("rate_req_per_s", "duration_s", "requests", "latency_p50_ms",
+                                              "latency_p99_ms", "latency_max_ms", "pp_per_s", "precision_counts")}
+        extra["latency_p50_ms"] = sv["latency_p50_ms"]
+        extra["latency_p99_ms"] = sv["latency_p99_ms"]
+        log(f"serving {args.serve_rate}/s for {args.serve_seconds}s: p50 {sv['latency_p50_ms']:.2f} ms, "
+            f"p99 {sv['latency_p99_ms']:.2f} ms, {sv['pp_per_s']:.3e} pp/s")
 
     cpu = None
     if not args.no_cpu:

@@ -62,7 +62,7 @@ def test_preempt_and_resume_is_bit_identical(gpu, world):
     sc, ds, _ = world
     (req,) = patch_requests(sc, 1, seed=5, w_range=(200, 200), h_range=(64, 64), sint_range=(3800, 3800),
                             precisions=("split",))
-    srv = Server(quantum_ms=0.05, pulse_chunk=64)  # ~64 pulses per round: many rounds
+    srv = Server(quantum_ms=0.001, pulse_chunk=64)  # one 64-pulse chunk per round: many rounds
     ref = srv.submit(ds, req)
     assert ref.wait(timeout=120)
     h = srv.submit(ds, req)
