@@ -29,8 +29,12 @@ class Handle:
         self.server, self.id, self.req, self.out, self.cnt, self.t_submit = server, rid, req, out, cnt, t_submit
 
     def wait(self, timeout=None) -> bool:
+        """True when the request is done; False on timeout or when it was preempted (see state());
+        raises when it failed."""
         st = _lib.load().sarbp_server_wait(self.server._h, self.id, -1.0 if timeout is None else float(timeout))
         if st == EAGAIN:
+            return False
+        if st == _lib.ESTATE and self.state() == _lib.REQ_PREEMPTED:
             return False
         check(st, "sarbp_server_wait")
         return True

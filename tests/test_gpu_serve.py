@@ -70,7 +70,7 @@ def test_preempt_and_resume_is_bit_identical(gpu, world):
     while h.info()["rounds"] < 3 and time.time() - t0 < 60:
         time.sleep(0.001)
     h.preempt()
-    h.wait(timeout=60)
+    assert not h.wait(timeout=60)  # returns at the round boundary where the state was saved
     inf = h.info()
     print("case preempt", inf, flush=True)
     assert inf["state"] == _lib.REQ_PREEMPTED and inf["preemptions"] == 1
