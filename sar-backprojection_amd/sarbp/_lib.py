@@ -5,8 +5,10 @@ instead of falling back to anything else.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import weakref
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libsarbp.so")
@@ -98,6 +100,30 @@ SIGNATURES = {
 }
 
 _lib = None
+
+# Objects holding library handles.  They are closed at interpreter exit, before the HIP runtime
+# tears itself down; finalizers that run later do nothing.
+_live = weakref.WeakSet()
+_shutting_down = False
+
+
+def track(obj) -> None:
+    _live.add(obj)
+
+
+def alive() -> bool:
+    return not _shutting_down
+
+
+@atexit.register
+def _close_all():
+    global _shutting_down
+    for o in sorted(list(_live), key=lambda o: getattr(o, "_close_order", 0)):
+        try:
+            o.close()
+        except Exception:
+            pass
+    _shutting_down = True
 
 
 class SarbpError(RuntimeError):

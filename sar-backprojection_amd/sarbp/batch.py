@@ -48,6 +48,7 @@ class Request:
 
 
 class Batch:
+    _close_order = 1  # servers, then batches/scenes, then datasets
     def __init__(self, ds: Dataset, requests):
         self.ds = ds
         self.requests = list(requests)
@@ -55,6 +56,7 @@ class Batch:
         h = ctypes.c_void_p()
         check(_lib.load().sarbp_batch_create(ds.handle, arr, len(self.requests), ctypes.byref(h)), "sarbp_batch_create")
         self._h = h
+        _lib.track(self)
 
     def run(self, tile_px: int = 0, pulse_chunk: int = 0) -> float:
         """One persistent launch over every request; returns the kernel time in ms."""
@@ -88,6 +90,8 @@ class Batch:
             self._h = None
 
     def __del__(self):
+        if not _lib.alive():
+            return
         try:
             self.close()
         except Exception:

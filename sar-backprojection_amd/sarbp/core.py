@@ -37,6 +37,7 @@ def raw_colmajor(raw) -> np.ndarray:
 
 
 class Dataset:
+    _close_order = 2  # servers, then batches/scenes, then datasets
     """RawData, trajectory and the three scalars, resident in HBM (sarbp_dataset_create)."""
 
     def __init__(self, raw, tx, ty, tz, near_range, dra, lambda_val):
@@ -53,6 +54,7 @@ class Dataset:
                                      tz.ctypes.data, self.near_range, self.dra, self.lambda_val,
                                      ctypes.byref(h)), "sarbp_dataset_create")
         self._h = h
+        _lib.track(self)
 
     @classmethod
     def streamed(cls, fill, nra, naz, tx, ty, tz, near_range, dra, lambda_val):
@@ -87,6 +89,7 @@ class Dataset:
             raise errors[0]
         check(st, "sarbp_dataset_create_streamed")
         self._h = h
+        _lib.track(self)
         return self
 
     @property
@@ -106,6 +109,8 @@ class Dataset:
             self._h = None
 
     def __del__(self):
+        if not _lib.alive():
+            return
         try:
             self.close()
         except Exception:
@@ -137,6 +142,7 @@ class Dataset:
 
 
 class Scene:
+    _close_order = 1  # servers, then batches/scenes, then datasets
     """A device-resident pixel set (sarbp_scene_create) focused repeatedly against one dataset."""
 
     def __init__(self, ds: Dataset, x, y, z, mask, index, sint, dimx=0):
@@ -154,6 +160,7 @@ class Scene:
                                              self.mask.ctypes.data, self.index.ctypes.data, self.sint.ctypes.data,
                                              self.n, self.dimx, ctypes.byref(h)), "sarbp_scene_create")
         self._h = h
+        _lib.track(self)
 
     def set_tiling(self, precision="f64", kernel=1, tile_x=16, accum=0):
         """Work mapping for one precision (sarbp_scene_set_tiling)."""
@@ -184,6 +191,8 @@ class Scene:
             self._h = None
 
     def __del__(self):
+        if not _lib.alive():
+            return
         try:
             self.close()
         except Exception:

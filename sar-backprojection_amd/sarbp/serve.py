@@ -60,6 +60,7 @@ class Handle:
 
 
 class Server:
+    _close_order = 0  # servers, then batches/scenes, then datasets
     def __init__(self, hbm_budget_bytes=0, quantum_ms=0.0, max_active=0, pulse_chunk=0, phase_err_limit_rad=0.0):
         _lib.require_device()
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), float(quantum_ms), int(max_active), int(pulse_chunk),
@@ -67,6 +68,7 @@ class Server:
         h = ctypes.c_void_p()
         check(_lib.load().sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)), "sarbp_server_create")
         self._h = h
+        _lib.track(self)
 
     def submit(self, ds, req: Request) -> Handle:
         out = np.zeros(req.n, np.complex128)
@@ -90,6 +92,8 @@ class Server:
             self._h = None
 
     def __del__(self):
+        if not _lib.alive():
+            return
         try:
             self.close()
         except Exception:
