@@ -163,6 +163,73 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 // LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
 // stages records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for
 // ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
+// Second half of one split term: phase, sin/cos, weight, complex MAC.  Applied one pulse behind
+// the first half (split_span_t), so the RawData gather of pulse k+1 is in flight while pulse k is
+// accumulated; the accumulation order stays ascending.
+template <int ACCUM>
+__device__ __forceinline__ void split_mac(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
+                                          float beta_t, float &sr, float &si, float &cr, float &ci) {
+  const float ph = c1 * dR;
+  const float phl = fmaf(c1, dR, -ph);  // exact residual of the product
+  const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, t0) + phl);
+  const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
+  const float cs = __builtin_amdgcn_cosf(tt);
+  const float w = fmaf(dR, two_r0 + dR, r0sq);  // R^2
+  const float wr = w * d.x, wi = w * d.y;
+  const float tre = fmaf(wr, cs, wi * sn);      // R^2 d exp(-j phi)
+  const float tim = fmaf(wi, cs, -(wr * sn));
+  if (ACCUM == 0) {
+    const float yr = tre - cr, yi = tim - ci;
+    const float nr = sr + yr, ni = si + yi;
+    cr = (nr - sr) - yr;
+    ci = (ni - si) - yi;
+    sr = nr;
+    si = ni;
+  } else {
+    sr += tre;
+    si += tim;
+  }
+}
+
+// First half of one split term for pulse a (record rk): range offset, range bin, and the RawData
+// gather.  go = the pulse contributes (inside this pixel's aperture, bin in [0, Nra-1)).
+struct SplitHalf {
+  bool go;
+  float dR, t0, c1, r0sq, two_r0;
+  float2 d;
+};
+
+template <bool FOURTH>
+__device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, const SplitTile &s,
+                                            const PulseRec *rk, int a, float inv_dra, unsigned nbins,
+                                            SplitHalf &h) {
+  const float4 ra = rk->a, rb = rk->b, rc = rk->c;
+  h.t0 = rb.w;
+  h.c1 = rc.x;
+  h.r0sq = rc.y;
+  h.two_r0 = rc.z;
+  const bool act = a >= t.lo && a < t.hi;
+  // computed on every lane (lanes outside their aperture have zero offsets: finite, unused)
+  const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
+  const float t2 = fmaf(-av, av, s.qo);
+  float dR = fmaf(t2, ra.w, av);
+  dR = fmaf(-(av * t2), rb.x, dR);
+  if (FOURTH) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
+  h.dR = dR;
+  const float qb = fmaf(dR, inv_dra, rb.y);
+  const float rq = __builtin_rintf(qb);
+  int bin = __float_as_int(rb.z) + (int)rq;
+  if (act && fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly (rare)
+    const double4 tr = g.traj[a];
+    bin = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);
+  }
+  h.go = act && (unsigned)bin < nbins;
+  const float2 *col = g.raw + (int64_t)a * g.nra;  // wave-uniform column of pulse a
+  h.d = col[h.go ? (unsigned)bin : 0u];            // every lane loads: no branch around the gather
+}
+
+// Pulses are processed two at a time: both gathers are in flight before either accumulation, which
+// stays in ascending pulse order.
 template <int ACCUM, bool FOURTH>
 __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                              int p1, int rec0, int C, float &sr, float &si, float &cr,
@@ -173,6 +240,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
   const float beta_t = (float)(g.beta * INV_2PI);
   const unsigned nbins = (unsigned)(g.nra - 1);
   const int nrec = C << s.ty_shift;
+  const int rstep = 1 << s.ty_shift;
   for (; c0 < p1; c0 += C) {
     for (int m = lane; m < nrec; m += 64) {
       const int p = c0 + (m >> s.ty_shift);
@@ -182,51 +250,20 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
-    for (int k = 0; k < cn; ++k) {
-      const int a = c0 + k;
-      const int ri = rec0 + s.row + (k << s.ty_shift);
-      const float4 ra = rec[ri].a, rb = rec[ri].b, rc = rec[ri].c;
-      if (a >= t.lo && a < t.hi) {
-        const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
-        const float t2 = fmaf(-av, av, s.qo);
-        float dR = fmaf(t2, ra.w, av);
-        dR = fmaf(-(av * t2), rb.x, dR);
-        if (FOURTH) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
-        const float qb = fmaf(dR, inv_dra, rb.y);
-        const float rq = __builtin_rintf(qb);
-        int bin;
-        if (fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly
-          const double4 tr = g.traj[a];
-          bin = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);
-        } else {
-          bin = __float_as_int(rb.z) + (int)rq;
-        }
-        if ((unsigned)bin < nbins) {
-          const float2 *col = g.raw + (int64_t)a * g.nra;  // wave-uniform column of pulse a
-          const float2 d = col[(unsigned)bin];
-          const float ph = rc.x * dR;
-          const float phl = fmaf(rc.x, dR, -ph);  // exact residual of the product
-          const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, rb.w) + phl);
-          const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
-          const float cs = __builtin_amdgcn_cosf(tt);
-          const float w = fmaf(dR, rc.z + dR, rc.y);  // R^2
-          const float wr = w * d.x, wi = w * d.y;
-          const float tre = fmaf(wr, cs, wi * sn);    // R^2 d exp(-j phi)
-          const float tim = fmaf(wi, cs, -(wr * sn));
-          if (ACCUM == 0) {
-            const float yr = tre - cr, yi = tim - ci;
-            const float nr = sr + yr, ni = si + yi;
-            cr = (nr - sr) - yr;
-            ci = (ni - si) - yi;
-            sr = nr;
-            si = ni;
-          } else {
-            sr += tre;
-            si += tim;
-          }
-          ++cnt;
-        }
-      }
+    const PulseRec *rk = rec + rec0 + s.row;
+    SplitHalf X, Y;
+    int k = 0;
+    for (; k + 1 < cn; k += 2, rk += 2 * rstep) {
+      split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
+      split_first<FOURTH>(g, t, s, rk + rstep, c0 + k + 1, inv_dra, nbins, Y);
+      if (X.go) split_mac<ACCUM>(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, sr, si, cr, ci);
+      if (Y.go) split_mac<ACCUM>(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t, sr, si, cr, ci);
+      cnt += (X.go ? 1 : 0) + (Y.go ? 1 : 0);
+    }
+    if (k < cn) {  // odd tail
+      split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
+      if (X.go) split_mac<ACCUM>(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, sr, si, cr, ci);
+      cnt += X.go ? 1 : 0;
     }
     if (ACCUM == 1) {
       ar += (double)sr;
