@@ -163,31 +163,33 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 // LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
 // stages records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for
 // ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
-// Second half of one split term: phase, sin/cos, weight, complex MAC.  Applied one pulse behind
-// the first half (split_span_t), so the RawData gather of pulse k+1 is in flight while pulse k is
-// accumulated; the accumulation order stays ascending.
-template <int ACCUM>
-__device__ __forceinline__ void split_mac(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
-                                          float beta_t, float &sr, float &si, float &cr, float &ci) {
+// Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi); exactly
+// zero when the pulse does not contribute (computed on every lane: no branch in the hot loop).
+__device__ __forceinline__ float2 split_term(bool go, float dR, float2 d, float t0, float c1, float r0sq,
+                                             float two_r0, float beta_t) {
   const float ph = c1 * dR;
   const float phl = fmaf(c1, dR, -ph);  // exact residual of the product
   const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, t0) + phl);
   const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
   const float cs = __builtin_amdgcn_cosf(tt);
-  const float w = fmaf(dR, two_r0 + dR, r0sq);  // R^2
+  const float w = go ? fmaf(dR, two_r0 + dR, r0sq) : 0.f;  // R^2, or 0 for a pulse that does not count
   const float wr = w * d.x, wi = w * d.y;
-  const float tre = fmaf(wr, cs, wi * sn);      // R^2 d exp(-j phi)
-  const float tim = fmaf(wi, cs, -(wr * sn));
+  return make_float2(fmaf(wr, cs, wi * sn), fmaf(wi, cs, -(wr * sn)));
+}
+
+// Compensated accumulation of one (pair) sum: Kahan (ACCUM 0) or plain FP32 within a chunk (ACCUM 1).
+template <int ACCUM>
+__device__ __forceinline__ void split_acc(float2 t, float &sr, float &si, float &cr, float &ci) {
   if (ACCUM == 0) {
-    const float yr = tre - cr, yi = tim - ci;
+    const float yr = t.x - cr, yi = t.y - ci;
     const float nr = sr + yr, ni = si + yi;
     cr = (nr - sr) - yr;
     ci = (ni - si) - yi;
     sr = nr;
     si = ni;
   } else {
-    sr += tre;
-    si += tim;
+    sr += t.x;
+    si += t.y;
   }
 }
 
@@ -228,8 +230,11 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
   h.d = col[h.go ? (unsigned)bin : 0u];            // every lane loads: no branch around the gather
 }
 
-// Pulses are processed two at a time: both gathers are in flight before either accumulation, which
-// stays in ascending pulse order.
+// Pulses are processed two at a time: both gathers are in flight before either term is formed; the
+// two terms are added and the pair sum is accumulated, pairs in ascending pulse order.  Pairs start
+// at even offsets from the chunk start, and chunks and caller ranges start at multiples of the
+// (even) chunk length from the union start, so the pairing -- and the result -- does not depend on
+// how a caller splits the pulse range.
 template <int ACCUM, bool FOURTH>
 __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                              int p1, int rec0, int C, float &sr, float &si, float &cr,
@@ -256,13 +261,15 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     for (; k + 1 < cn; k += 2, rk += 2 * rstep) {
       split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
       split_first<FOURTH>(g, t, s, rk + rstep, c0 + k + 1, inv_dra, nbins, Y);
-      if (X.go) split_mac<ACCUM>(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, sr, si, cr, ci);
-      if (Y.go) split_mac<ACCUM>(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t, sr, si, cr, ci);
+      const float2 tx = split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
+      const float2 ty = split_term(Y.go, Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
+      // pulses c0+k and c0+k+1 (k even from the chunk start) are summed in pairs, then compensated
+      split_acc<ACCUM>(make_float2(tx.x + ty.x, tx.y + ty.y), sr, si, cr, ci);
       cnt += (X.go ? 1 : 0) + (Y.go ? 1 : 0);
     }
-    if (k < cn) {  // odd tail
+    if (k < cn) {  // odd tail (only at the end of the pulse range)
       split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
-      if (X.go) split_mac<ACCUM>(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, sr, si, cr, ci);
+      split_acc<ACCUM>(split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt += X.go ? 1 : 0;
     }
     if (ACCUM == 1) {
