@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
     ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
-    ap.add_argument("--split-map", default="32,0", help="split work mapping tile_x,accum")
+    ap.add_argument("--split-map", default="16,0", help="split work mapping tile_x,accum")
     ap.add_argument("--sweep", action="store_true", help="time every work mapping first and use the fastest")
     ap.add_argument("--batch-requests", type=int, default=64, help="sub-scene requests of the batch leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson serving leg duration (0: skip)")
@@ -208,7 +208,7 @@ def main():
     traffic, traffic_src = None, None
     tf = os.path.join(ROOT, "profiles", "traffic_full-4000x250x40001.json")
     kname = {"split": "k_bp_split_tile<0>", "f64": "k_bp_f64_tile"}[head]
-    default_map = (head == "split" and args.split_map == "32,0") or (head == "f64" and args.f64_map == "1,8")
+    default_map = (head == "split" and args.split_map == "16,0") or (head == "f64" and args.f64_map == "1,8")
     if not args.small and default_map and not args.sweep and os.path.exists(tf):
         with open(tf) as f:
             tj = json.load(f)

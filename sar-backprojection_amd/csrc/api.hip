@@ -112,9 +112,9 @@ int sarbp_hip_fail(hipError_t e, const char *what) {
 Tiling default_tiling(int precision) {
   Tiling t;
   // fastest measured on the 4000 x 250 x 40001 workload: 8 x 8 tiles for FP64 (no per-row
-  // records to stage), 32 x 2 for the split path (two row reference ranges per pulse)
+  // records to stage), 16 x 4 for the split path (four row reference ranges per pulse)
   t.kernel = 1;
-  t.tile_x = precision == SARBP_PREC_SPLIT ? 32 : 8;
+  t.tile_x = precision == SARBP_PREC_SPLIT ? 16 : 8;
   t.accum = 0;
   return t;
 }

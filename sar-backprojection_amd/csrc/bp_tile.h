@@ -230,11 +230,11 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
   h.d = col[h.go ? (unsigned)bin : 0u];            // every lane loads: no branch around the gather
 }
 
-// Pulses are processed two at a time: both gathers are in flight before either term is formed; the
-// two terms are added and the pair sum is accumulated, pairs in ascending pulse order.  Pairs start
-// at even offsets from the chunk start, and chunks and caller ranges start at multiples of the
-// (even) chunk length from the union start, so the pairing -- and the result -- does not depend on
-// how a caller splits the pulse range.
+// Pulses are processed four at a time: the four gathers are in flight before any term is formed;
+// the terms are summed pairwise and the group sum is accumulated, groups in ascending pulse order.
+// Groups start at multiples of 4 from the chunk start, and chunks and caller ranges start at
+// multiples of the chunk length (a multiple of 8) from the union start, so the grouping -- and the
+// result -- does not depend on how a caller splits the pulse range.
 template <int ACCUM, bool FOURTH>
 __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                              int p1, int rec0, int C, float &sr, float &si, float &cr,
@@ -256,18 +256,23 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row;
-    SplitHalf X, Y;
+    SplitHalf X, Y, Z, W;
     int k = 0;
-    for (; k + 1 < cn; k += 2, rk += 2 * rstep) {
+    for (; k + 3 < cn; k += 4, rk += 4 * rstep) {
       split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
       split_first<FOURTH>(g, t, s, rk + rstep, c0 + k + 1, inv_dra, nbins, Y);
+      split_first<FOURTH>(g, t, s, rk + 2 * rstep, c0 + k + 2, inv_dra, nbins, Z);
+      split_first<FOURTH>(g, t, s, rk + 3 * rstep, c0 + k + 3, inv_dra, nbins, W);
       const float2 tx = split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
       const float2 ty = split_term(Y.go, Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
-      // pulses c0+k and c0+k+1 (k even from the chunk start) are summed in pairs, then compensated
-      split_acc<ACCUM>(make_float2(tx.x + ty.x, tx.y + ty.y), sr, si, cr, ci);
-      cnt += (X.go ? 1 : 0) + (Y.go ? 1 : 0);
+      const float2 tz = split_term(Z.go, Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
+      const float2 tw = split_term(W.go, W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
+      // pulses c0+k .. c0+k+3 (k a multiple of 4 from the chunk start) are summed pairwise, then
+      // the group sum is accumulated with compensation
+      split_acc<ACCUM>(make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y)), sr, si, cr, ci);
+      cnt += (X.go ? 1 : 0) + (Y.go ? 1 : 0) + (Z.go ? 1 : 0) + (W.go ? 1 : 0);
     }
-    if (k < cn) {  // odd tail (only at the end of the pulse range)
+    for (; k < cn; ++k, rk += rstep) {  // tail (only at the end of the pulse range)
       split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
       split_acc<ACCUM>(split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt += X.go ? 1 : 0;
