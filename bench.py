@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
     ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
-    ap.add_argument("--split-map", default="16,0", help="split work mapping tile_x,accum[,kernel]")
+    ap.add_argument("--split-map", default="16,0", help="split work mapping tile_x,accum")
     ap.add_argument("--sweep", action="store_true", help="time every work mapping first and use the fastest")
     ap.add_argument("--batch-requests", type=int, default=64, help="sub-scene requests of the batch leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson serving leg duration (0: skip)")
@@ -119,14 +119,13 @@ def main():
     log(f"dataset resident ({ds.device_bytes / 1e9:.2f} GB), {scene.n} pixels, {pp} pp")
     k, tx = (int(v) for v in args.f64_map.split(","))
     scene.set_tiling("f64", k, tx, 0)
-    sm = [int(v) for v in args.split_map.split(",")]
-    tx, acc = sm[0], sm[1]
-    scene.set_tiling("split", sm[2] if len(sm) > 2 else 1, tx, acc)
+    tx, acc = (int(v) for v in args.split_map.split(","))
+    scene.set_tiling("split", 1, tx, acc)
     sweep = {}
     if args.sweep:
         best = {}
         for prec, maps in (("f64", [(0, 16, 0)] + [(1, t, 0) for t in (64, 32, 16, 8, 4)]),
-                           ("split", [(kk, t, a) for kk in (1, 2) for t in (64, 32, 16, 8) for a in (0, 1)])):
+                           ("split", [(1, t, a) for t in (64, 32, 16, 8) for a in (0, 1)])):
             for m in maps:
                 scene.set_tiling(prec, *m)
                 _, ks = time_precision(scene, prec, 2, 1)

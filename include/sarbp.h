@@ -87,8 +87,8 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
 /*
  * Work mapping of one precision (tuning; the defaults need no call).  kernel 1 = one 64-pixel
  * tile per wavefront walking the union of the tile's apertures pulse by pulse, tile_x x (64/tile_x)
- * pixels; kernel 0 = one pixel per lane (FP64 only); kernel 2 = the split tile kernel built for 8
- * wavefronts per SIMD.  accum (split only): 0 = Kahan FP32, 1 = FP32 chunk sums folded into FP64.
+ * pixels; kernel 0 = one pixel per lane (FP64 only).  accum (split only): 0 = Kahan FP32,
+ * 1 = FP32 blocks of 64 pulses folded into FP64.
  */
 int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_x, int accum);
 int sarbp_scene_destroy(sarbp_scene *sc);

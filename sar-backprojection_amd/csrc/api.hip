@@ -339,12 +339,10 @@ int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_
     return sarbp_fail(SARBP_EINVAL, "unknown precision");
   if (tile_x != 64 && tile_x != 32 && tile_x != 16 && tile_x != 8 && tile_x != 4)
     return sarbp_fail(SARBP_EINVAL, "tile_x must be 64, 32, 16, 8 or 4");
-  if (kernel < 0 || kernel > 2 || accum < 0 || accum > 1)
-    return sarbp_fail(SARBP_EINVAL, "kernel must be 0, 1 or 2 and accum 0 or 1");
+  if (kernel < 0 || kernel > 1 || accum < 0 || accum > 1)
+    return sarbp_fail(SARBP_EINVAL, "kernel and accum must be 0 or 1");
   if (precision == SARBP_PREC_SPLIT && kernel == 0)
     return sarbp_fail(SARBP_EINVAL, "the split path always uses tiles");
-  if (precision == SARBP_PREC_F64 && kernel == 2)
-    return sarbp_fail(SARBP_EINVAL, "kernel 2 exists for the split path only");
   Tiling &t = precision == SARBP_PREC_SPLIT ? sc->tiling_split : sc->tiling_f64;
   t.kernel = kernel;
   t.tile_x = tile_x;

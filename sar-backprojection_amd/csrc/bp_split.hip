@@ -9,8 +9,8 @@ namespace {
 constexpr int SPLIT_SCENE_RECS = 128;
 
 template <int ACCUM>
-__device__ __forceinline__ void split_tile_body(const BpGeom &g, const PixelsView &px, const TileGrid &tg,
-                                                double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
+__global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
+                                                      double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
   const int C = SPLIT_SCENE_RECS / tg.ty;  // LDS: SPLIT_SCENE_RECS records (dynamic)
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= tg.ntiles()) return;  // whole wavefront
@@ -30,20 +30,6 @@ __device__ __forceinline__ void split_tile_body(const BpGeom &g, const PixelsVie
   if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 
-template <int ACCUM>
-__global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
-                                                      double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
-  split_tile_body<ACCUM>(g, px, tg, out, ncnt);
-}
-
-// Same, with the register budget capped for 8 wavefronts per SIMD (more latency hiding, a few
-// registers spilled).
-template <int ACCUM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_bp_split_tile_o8(
-    BpGeom g, PixelsView px, TileGrid tg, double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
-  split_tile_body<ACCUM>(g, px, tg, out, ncnt);
-}
-
 }  // namespace
 
 int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st) {
@@ -51,14 +37,7 @@ int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Til
   const TileGrid tg = make_tile_grid(sc->dimx, sc->dimy, t.tile_x);
   const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
   if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
-  const size_t lds = SPLIT_SCENE_RECS * sizeof(PulseRec);
-  if (t.kernel == 2 && t.accum == 0)
-    hipLaunchKernelGGL(k_bp_split_tile_o8<0>, dim3((unsigned)blocks), dim3(64), lds, st, ds->geom(), sc->view(), tg,
-                       sc->out, sc->ncnt);
-  else if (t.kernel == 2)
-    hipLaunchKernelGGL(k_bp_split_tile_o8<1>, dim3((unsigned)blocks), dim3(64), lds, st, ds->geom(), sc->view(), tg,
-                       sc->out, sc->ncnt);
-  else if (t.accum == 0)
+  if (t.accum == 0)
     hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, ds->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   else

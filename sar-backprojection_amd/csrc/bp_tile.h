@@ -165,9 +165,8 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 // ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
 // Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi); exactly
 // zero when the pulse does not contribute (computed on every lane: no branch in the hot loop).
-__device__ __forceinline__ float2 split_term(bool go, float dR, float2 d, const PulseRec *rk, float beta_t) {
-  // the record's uniform coefficients are read again here rather than kept in registers
-  const float t0 = rk->b.w, c1 = rk->c.x, r0sq = rk->c.y, two_r0 = rk->c.z;
+__device__ __forceinline__ float2 split_term(bool go, float dR, float2 d, float t0, float c1, float r0sq,
+                                             float two_r0, float beta_t) {
   const float ph = c1 * dR;
   const float phl = fmaf(c1, dR, -ph);  // exact residual of the product
   const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, t0) + phl);
@@ -198,7 +197,7 @@ __device__ __forceinline__ void split_acc(float2 t, float &sr, float &si, float 
 // gather.  go = the pulse contributes (inside this pixel's aperture, bin in [0, Nra-1)).
 struct SplitHalf {
   bool go;
-  float dR;
+  float dR, t0, c1, r0sq, two_r0;
   float2 d;
 };
 
@@ -207,6 +206,10 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
                                             const PulseRec *rk, int a, float inv_dra, unsigned nbins,
                                             SplitHalf &h) {
   const float4 ra = rk->a, rb = rk->b, rc = rk->c;
+  h.t0 = rb.w;
+  h.c1 = rc.x;
+  h.r0sq = rc.y;
+  h.two_r0 = rc.z;
   const bool act = a >= t.lo && a < t.hi;
   // computed on every lane (lanes outside their aperture have zero offsets: finite, unused)
   const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
@@ -260,10 +263,10 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
       split_first<FOURTH>(g, t, s, rk + rstep, c0 + k + 1, inv_dra, nbins, Y);
       split_first<FOURTH>(g, t, s, rk + 2 * rstep, c0 + k + 2, inv_dra, nbins, Z);
       split_first<FOURTH>(g, t, s, rk + 3 * rstep, c0 + k + 3, inv_dra, nbins, W);
-      const float2 tx = split_term(X.go, X.dR, X.d, rk, beta_t);
-      const float2 ty = split_term(Y.go, Y.dR, Y.d, rk + rstep, beta_t);
-      const float2 tz = split_term(Z.go, Z.dR, Z.d, rk + 2 * rstep, beta_t);
-      const float2 tw = split_term(W.go, W.dR, W.d, rk + 3 * rstep, beta_t);
+      const float2 tx = split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
+      const float2 ty = split_term(Y.go, Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
+      const float2 tz = split_term(Z.go, Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
+      const float2 tw = split_term(W.go, W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
       // pulses c0+k .. c0+k+3 (k a multiple of 4 from the chunk start) are summed pairwise, then
       // the group sum is accumulated with compensation
       split_acc<ACCUM>(make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y)), sr, si, cr, ci);
@@ -271,7 +274,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     }
     for (; k < cn; ++k, rk += rstep) {  // tail (only at the end of the pulse range)
       split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
-      split_acc<ACCUM>(split_term(X.go, X.dR, X.d, rk, beta_t), sr, si, cr, ci);
+      split_acc<ACCUM>(split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt += X.go ? 1 : 0;
     }
     if (ACCUM == 1) {
