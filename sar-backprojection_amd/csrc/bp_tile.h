@@ -226,8 +226,10 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
     bin = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);
   }
   h.go = act && (unsigned)bin < nbins;
+  // every lane loads (no branch around the gather); a lane that does not count reads its clamped
+  // bin, which for a lane outside its aperture is its row's reference bin: the same cache lines
   const float2 *col = g.raw + (int64_t)a * g.nra;  // wave-uniform column of pulse a
-  h.d = col[h.go ? (unsigned)bin : 0u];            // every lane loads: no branch around the gather
+  h.d = col[min((unsigned)bin, nbins - 1u)];
 }
 
 // Pulses are processed four at a time: the four gathers are in flight before any term is formed;
