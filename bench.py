@@ -60,26 +60,69 @@ def time_precision(scene, precision, steps, warmup):
     return float(np.mean(walls)), float(np.mean(gpus)) / 1e3
 
 
-def cpu_baseline(sc, strip_cols, gpu_strip):
+def cpu_baseline(sc, strip_cols, serial_cols, gpu_strip):
     """Oracle (C restatement) on a bounded strip of the same workload, all host cores, compared
-    against the GPU result of the same pixels."""
+    against the GPU result of the same pixels; plus the serial (1-thread) rate on a narrower strip,
+    the reference's own loop structure (sar_focuser_puntual.py:259)."""
     from oracle import oracle_lib as O
     x0 = sc.shape[0] // 2 - strip_cols // 2
     st = sc.sub(x0, x0 + strip_cols)
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     cores = min(cores, 16)
-    pp = O.count_pp(*[np.asarray(a).flatten(order="F") for a in (st.mask, st.index, st.sint)], st.Naz)
+    cnt = lambda s: O.count_pp(*[np.asarray(a).flatten(order="F") for a in (s.mask, s.index, s.sint)], s.Naz)  # noqa
+    pp = cnt(st)
     t0 = time.perf_counter()
     ref, rcnt = O.focus(*st.args(), nthreads=cores)
     dt = time.perf_counter() - t0
+    s1 = sc.sub(x0, x0 + serial_cols)
+    pp1 = cnt(s1)
+    t1 = time.perf_counter()
+    O.focus(*s1.args(), nthreads=1)
+    dt1 = time.perf_counter() - t1
+    base = {"value": pp / dt, "unit": "pixel·pulses/s", "cores": int(cores), "kind": "port",
+            "sample": f"{strip_cols}x{sc.shape[1]} px strip (x {x0}..{x0 + strip_cols - 1}) of the same "
+                      f"workload, {pp} pp, {dt:.2f} s on {cores} threads, C restatement of process_point "
+                      f"(OpenMP over pixels)",
+            "serial_value": pp1 / dt1,
+            "serial_sample": f"{serial_cols}x{sc.shape[1]} px strip, {pp1} pp, {dt1:.2f} s, 1 thread"}
     got = gpu_strip(x0, x0 + strip_cols)
+    if got is None:
+        return base, {}
     nz = rcnt.ravel() > 0
     rel = np.abs(got.ravel()[nz] - ref.ravel()[nz]) / np.abs(ref.ravel()[nz])
     zeros_ok = bool(np.all(got.ravel()[~nz] == 0))
-    return {"value": pp / dt, "unit": "pixel·pulses/s", "cores": int(cores), "kind": "port",
-            "sample": f"{strip_cols}x{sc.shape[1]} px strip (x {x0}..{x0 + strip_cols - 1}) of the same "
-                      f"workload, {pp} pp, {dt:.2f} s, C restatement of process_point with OpenMP"}, \
-        {"strip_f64_max_rel_err": float(rel.max()) if nz.any() else 0.0, "strip_zero_pixels_exact": zeros_ok}
+    return base, {"strip_f64_max_rel_err": float(rel.max()) if nz.any() else 0.0,
+                  "strip_zero_pixels_exact": zeros_ok, "strip_pixels_checked": int(nz.sum())}
+
+
+def target_peaks(sc, img_split, img_f64, half_x=3, half_y=2):
+    """Point-target responses: for each simulated target the peak of |image| in a
+    window around its pixel, split vs FP64: same peak pixel, relative peak-height difference."""
+    a_s = np.abs(img_split.reshape(sc.shape, order="F"))
+    a_d = np.abs(img_f64.reshape(sc.shape, order="F"))
+    nx, ny = sc.shape
+    dx = float(sc.x[1, 0] - sc.x[0, 0])
+    dy = float(sc.y[0, 1] - sc.y[0, 0])
+    med = float(np.median(a_d))
+    same, rel, used = 0, [], 0
+    for tx_, ty_ in sc.targets[:, :2]:
+        ix = int(round((tx_ - sc.x[0, 0]) / dx))
+        iy = int(round((ty_ - sc.y[0, 0]) / dy))
+        if ix - half_x < 0 or ix + half_x >= nx or iy - half_y < 0 or iy + half_y >= ny:
+            continue
+        ws = a_s[ix - half_x:ix + half_x + 1, iy - half_y:iy + half_y + 1]
+        wd = a_d[ix - half_x:ix + half_x + 1, iy - half_y:iy + half_y + 1]
+        if wd.max() <= 3 * med:  # target below the noise (or outside every aperture)
+            continue
+        used += 1
+        ps, pd = np.unravel_index(np.argmax(ws), ws.shape), np.unravel_index(np.argmax(wd), wd.shape)
+        same += int(ps == pd)
+        rel.append(abs(float(ws.max()) / float(wd.max()) - 1.0))
+    rel = np.asarray(rel)
+    return {"targets": int(sc.targets.shape[0]), "targets_above_noise": used,
+            "same_peak_pixel_fraction": same / max(used, 1),
+            "peak_height_rel_err_rms": float(np.sqrt(np.mean(rel ** 2))) if used else None,
+            "peak_height_rel_err_max": float(rel.max()) if used else None}
 
 
 def main():
@@ -88,7 +131,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--strip", type=int, default=256, help="pixel columns of the CPU baseline strip")
+    ap.add_argument("--strip", type=int, default=1024, help="pixel columns of the all-cores CPU baseline strip")
+    ap.add_argument("--serial-strip", type=int, default=64, help="pixel columns of the 1-thread CPU strip")
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
     ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
@@ -159,6 +203,8 @@ def main():
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
         extra["split_vs_f64_errors"] = phase_amp_errors(outs["split"], outs["f64"])
+        if sc.targets is not None:
+            extra["split_vs_f64_target_peaks"] = target_peaks(sc, outs["split"], outs["f64"])
 
     # batch leg: uneven sub-scene requests in one persistent launch, mapping chosen by the autotuner
     if args.batch_requests > 0:
@@ -198,7 +244,7 @@ def main():
                 return None
             return full_f64.reshape(shape, order="F")[a:b]
 
-        cpu, parity = cpu_baseline(sc, args.strip, gpu_strip)
+        cpu, parity = cpu_baseline(sc, args.strip, args.serial_strip, gpu_strip)
         extra.update(parity)
         log(f"cpu baseline {cpu['value']:.4e} pp/s on {cpu['cores']} cores; strip parity {parity}")
 
