@@ -62,8 +62,8 @@ int dataset_init(int64_t nra, int64_t naz, const double *tx, const double *ty, c
   if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
   *out = nullptr;
   if (nra < 2 || naz < 3) return sarbp_fail(SARBP_EINVAL, "RawData must have nra >= 2 and naz >= 3");
-  if (nra > (int64_t)0x7fffffff || naz > (int64_t)0x7fffffff)
-    return sarbp_fail(SARBP_EINVAL, "nra and naz must each fit in int32");
+  if (nra > ((int64_t)1 << 24) || naz > (int64_t)0x7fffffff)
+    return sarbp_fail(SARBP_EINVAL, "nra must be at most 2^24 range bins and naz must fit in int32");
   if (!tx || !ty || !tz) return sarbp_fail(SARBP_EINVAL, "trajectory pointer is null");
   if (!(dra != 0.0) || !std::isfinite(dra) || !std::isfinite(near_range) || !(lambda_val != 0.0) ||
       !std::isfinite(lambda_val))

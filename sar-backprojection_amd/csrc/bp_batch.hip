@@ -45,7 +45,7 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
   SplitTile s;
   if (split) {
     s = split_tile(g, r.tg, t);
-    split = s.omax <= SPLIT_OMAX_M;  // not a compact tile: FP64 terms (same decision every range)
+    split = s.use;  // not a compact tile: FP64 terms (same decision every range)
   }
   if (!split) {
     f64_span(g, t, p0, p1, ar, ai, cnt);

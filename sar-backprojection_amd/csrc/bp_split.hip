@@ -19,7 +19,7 @@ __global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, T
   int32_t cnt = 0;
   if (t.wlo < t.whi) {
     const SplitTile s = split_tile(g, tg, t);
-    if (s.omax > SPLIT_OMAX_M) {  // not a compact tile: FP64 terms
+    if (!s.use) {  // not a compact tile: FP64 terms
       f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
     } else {
       float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;

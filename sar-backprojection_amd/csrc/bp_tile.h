@@ -24,6 +24,8 @@
 //   FP64 terms: the series and FP32 offsets are only accurate for compact tiles.
 #pragma once
 
+#include <climits>
+
 #include "bp_common.h"
 
 namespace {
@@ -80,15 +82,15 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const TileLane &t, int
 
 struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads)
   float4 a;        // ux, uy, uz, 1/(2R0)
-  float4 b;        // 1/(2R0^2), f0 = q0 - B0, B0 (int bits), t0 = frac(phi0/2pi)
+  float4 b;        // 1/(2R0^2), f0 = q0 - B0, 8*B0 (int bits: byte offset of bin B0), t0 = frac(phi0/2pi)
   float4 c;        // c1 = (k + 2 beta R0)/2pi, R0^2, 2R0, 1/(8R0^3)
 };
 
 // Dynamic LDS of every kernel that stages split records (indexed, so addressing stays 32-bit).
 extern __shared__ PulseRec sarbp_lds_rec[];
 
-__device__ __forceinline__ void make_rec(const BpGeom &g, double xc, double yc, double zc, double4 t,
-                                         PulseRec &r) {
+__device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, double zc, double4 t,
+                                        PulseRec &r) {
   const double dx = xc - t.x, dy = yc - t.y, dz = zc - t.z;
   const double r0sq = dx * dx + dy * dy + dz * dz;
   // 1/R0 from the hardware reciprocal square root and two Newton steps (a few ulp; R0 is only the
@@ -98,15 +100,18 @@ __device__ __forceinline__ void make_rec(const BpGeom &g, double xc, double yc, 
   inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
   inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
   const double r0 = r0sq * inv;
-  const double q0 = (r0 - g.near_range) * g.inv_dra;
+  // bin offset, kept within +-2^25 bins so that 8 x bin stays far from int32 wrap-around (a
+  // reference point that far outside the swath has no bin inside it either)
+  const double q0 = fmin(fmax((r0 - g.near_range) * g.inv_dra, -33554432.0), 33554432.0);
   const double b0 = __builtin_rint(q0);
   const double ph0 = g.kphase * r0 + g.beta * r0sq;
   const double turns = ph0 * INV_2PI;
   const double t0 = turns - __builtin_floor(turns);
   const double c1 = (g.kphase + 2.0 * g.beta * r0) * INV_2PI;
   r.a = make_float4((float)(dx * inv), (float)(dy * inv), (float)(dz * inv), (float)(0.5 * inv));
-  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float((int)b0), (float)t0);
+  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float((int)b0 * 8), (float)t0);
   r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
+  return (int)b0;
 }
 
 // Row-relative quantities of the split form.  Each row of the tile (TX lanes at one grid y) has
@@ -116,6 +121,10 @@ struct SplitTile {
   double sxc, syc, szc;  // reference point of the row this lane stages records for (row lane % TY)
   float ox, oy, oz, qo;  // offset of this lane's pixel from its row's reference point
   float omax;            // largest pixel-to-reference distance in the tile (uniform)
+  bool use;              // the tile is compact enough for the split form (uniform)
+  unsigned poison;       // 0 for a lane with a non-empty aperture, else SPLIT_NO_SAMPLE
+  int core_lo, core_hi;  // pulses inside every non-empty aperture of the tile (uniform; may be empty)
+  int margin;            // bins: a reference bin this far inside [0, Nra-1) keeps every pixel's bin inside
   float guard;           // |f0 + dR/dra - nearest integer| above this -> decide the bin in FP64
   int ty_shift;          // log2(rows per tile)
   int row;               // this lane's row
@@ -155,24 +164,27 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   s.omax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sqrtf((float)wave_max_f64((double)s.qo)))));
   // error of f0 + dR/dra in bins: f0 and the FP32 sum ~1e-7, dR ~1.2e-7 * |o| m; margin >= 3x
   s.guard = 0.5f - (2e-5f + 4e-7f * s.omax * (float)g.inv_dra);
+  // compact tiles only; and |dR|/dra well inside the int32 byte offsets of split_first
+  s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 16777216.0;
+  s.poison = act ? 0u : 0x80000000u;
+  s.core_lo = __builtin_amdgcn_readfirstlane(wave_max_i32(act ? t.lo : INT_MIN));
+  s.core_hi = __builtin_amdgcn_readfirstlane(wave_min_i32(act ? t.hi : INT_MAX));
+  // |bin - B0| <= |f0| + |dR|/dra + 1 (an exact re-decision) with |dR| <= 1.01 |o|
+  s.margin = s.use ? (int)((double)s.omax * g.inv_dra * 1.25) + 4 : INT_MAX;
   return s;
 }
 
-// Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
-// and staged records do not depend on where a caller splits the range.  The wavefront's C * TY
-// LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (C a multiple of 64; lane l
-// stages records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for
-// ACCUM 1 (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.
 // Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi); exactly
-// zero when the pulse does not contribute (computed on every lane: no branch in the hot loop).
-__device__ __forceinline__ float2 split_term(bool go, float dR, float2 d, float t0, float c1, float r0sq,
-                                             float two_r0, float beta_t) {
+// zero when the pulse does not contribute, whose sample d is zero (computed on every lane: no
+// branch in the hot loop).
+__device__ __forceinline__ float2 split_term(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
+                                             float beta_t) {
   const float ph = c1 * dR;
   const float phl = fmaf(c1, dR, -ph);  // exact residual of the product
   const float tt = __builtin_amdgcn_fractf(ph) + (fmaf(beta_t, dR * dR, t0) + phl);
   const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
   const float cs = __builtin_amdgcn_cosf(tt);
-  const float w = go ? fmaf(dR, two_r0 + dR, r0sq) : 0.f;  // R^2, or 0 for a pulse that does not count
+  const float w = fmaf(dR, two_r0 + dR, r0sq);  // R^2
   const float wr = w * d.x, wi = w * d.y;
   return make_float2(fmaf(wr, cs, wi * sn), fmaf(wi, cs, -(wr * sn)));
 }
@@ -194,42 +206,62 @@ __device__ __forceinline__ void split_acc(float2 t, float &sr, float &si, float 
 }
 
 // First half of one split term for pulse a (record rk): range offset, range bin, and the RawData
-// gather.  go = the pulse contributes (inside this pixel's aperture, bin in [0, Nra-1)).
+// gather.  The gather goes through a buffer resource of pulse a's column (base and size in scalar
+// registers) with a per-lane byte offset; a lane whose pulse does not count (outside its aperture,
+// or bin outside [0, Nra-1)) passes an offset beyond the column, which the buffer range check turns
+// into a zero sample without a memory access.
 struct SplitHalf {
-  bool go;
+  bool bad;  // inside the aperture but the bin is outside [0, Nra-1): not counted (rare)
   float dR, t0, c1, r0sq, two_r0;
   float2 d;
 };
 
-template <bool FOURTH>
+constexpr unsigned SPLIT_NO_SAMPLE = 0x80000000u;  // byte offset beyond every column (Nra <= 2^24)
+
+// Buffer resource of pulse a's RawData column (wave-uniform: scalar registers), bins [0, Nra-1):
+// the last bin, which never contributes (sar_focuser_puntual.py:106), reads as zero too.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_column(const BpGeom &g, int a) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(g.raw + (int64_t)a * g.nra), (short)0,
+                                           (int)((g.nra - 1) * (int64_t)sizeof(float2)), 0x00020000);
+}
+
+// FAST: every lane with an aperture is inside it for pulse a and every bin of the chunk is inside
+// the swath (the caller checks both), so no per-lane aperture or bin-range test is needed.
+template <bool FOURTH, bool FAST>
 __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                            const PulseRec *rk, int a, float inv_dra, unsigned nbins,
-                                            SplitHalf &h) {
+                                            const PulseRec *rk, int a, __amdgpu_buffer_rsrc_t col,
+                                            float inv_dra, unsigned nbin_bytes, SplitHalf &h) {
   const float4 ra = rk->a, rb = rk->b, rc = rk->c;
   h.t0 = rb.w;
   h.c1 = rc.x;
   h.r0sq = rc.y;
   h.two_r0 = rc.z;
-  const bool act = a >= t.lo && a < t.hi;
+  const bool act = FAST || (a >= t.lo && a < t.hi);  // FAST: lanes without an aperture are poisoned below
   // computed on every lane (lanes outside their aperture have zero offsets: finite, unused)
   const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
   const float t2 = fmaf(-av, av, s.qo);
-  float dR = fmaf(t2, ra.w, av);
-  dR = fmaf(-(av * t2), rb.x, dR);
-  if (FOURTH) dR = fmaf(-t2 * fmaf(-5.f * av, av, s.qo), rc.w, dR);
+  // dR = a + t2 (1/(2R0) - a/(2R0^2) [- (|o|^2 - 5a^2)/(8R0^3)])
+  float c = fmaf(-av, rb.x, ra.w);
+  if (FOURTH) c = fmaf(-fmaf(-5.f * av, av, s.qo), rc.w, c);
+  const float dR = fmaf(t2, c, av);
   h.dR = dR;
   const float qb = fmaf(dR, inv_dra, rb.y);
   const float rq = __builtin_rintf(qb);
-  int bin = __float_as_int(rb.z) + (int)rq;
+  // byte offset of bin B0 + rq (|bin| < 2^26: no wrap-around)
+  unsigned off = ((unsigned)(int)rq << 3) + (unsigned)__float_as_int(rb.z);
   if (act && fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly (rare)
     const double4 tr = g.traj[a];
-    bin = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);
+    off = (unsigned)exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z) << 3;  // -1 -> out of range
   }
-  h.go = act && (unsigned)bin < nbins;
-  // every lane loads (no branch around the gather); a lane that does not count reads its clamped
-  // bin, which for a lane outside its aperture is its row's reference bin: the same cache lines
-  const float2 *col = g.raw + (int64_t)a * g.nra;  // wave-uniform column of pulse a
-  h.d = col[min((unsigned)bin, nbins - 1u)];
+  if (FAST) {
+    h.bad = false;
+    off |= s.poison;  // beyond the column for a lane without an aperture
+  } else {
+    const bool ok = off < nbin_bytes;  // 0 <= bin < Nra-1
+    h.bad = act && !ok;
+    off = act && ok ? off : SPLIT_NO_SAMPLE;
+  }
+  h.d = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(col, (int)off, 0, 0));
 }
 
 // Pulses are processed four at a time: the four gathers are in flight before any term is formed;
@@ -237,6 +269,38 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
 // Groups start at multiples of 4 from the chunk start, and chunks and caller ranges start at
 // multiples of the chunk length (a multiple of 8) from the union start, so the grouping -- and the
 // result -- does not depend on how a caller splits the pulse range.
+template <int ACCUM, bool FOURTH, bool FAST>
+__device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
+                                            const PulseRec *rk, int rstep, int a, float inv_dra,
+                                            unsigned nbin_bytes, float beta_t, float &sr, float &si, float &cr,
+                                            float &ci, int32_t &cnt) {
+  const __amdgpu_buffer_rsrc_t c0r = raw_column(g, a), c1r = raw_column(g, a + 1), c2r = raw_column(g, a + 2),
+                               c3r = raw_column(g, a + 3);
+  SplitHalf X, Y, Z, W;
+  split_first<FOURTH, FAST>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
+  split_first<FOURTH, FAST>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
+  split_first<FOURTH, FAST>(g, t, s, rk + 2 * rstep, a + 2, c2r, inv_dra, nbin_bytes, Z);
+  split_first<FOURTH, FAST>(g, t, s, rk + 3 * rstep, a + 3, c3r, inv_dra, nbin_bytes, W);
+  const float2 tx = split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
+  const float2 ty = split_term(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
+  const float2 tz = split_term(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
+  const float2 tw = split_term(W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
+  // pulses a .. a+3 (a a multiple of 4 from the chunk start) are summed pairwise, then the group
+  // sum is accumulated with compensation
+  split_acc<ACCUM>(make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y)), sr, si, cr, ci);
+  if (!FAST && (X.bad || Y.bad || Z.bad || W.bad)) {  // rare: a bin outside the swath
+    __asm__ volatile("");  // keeps the branch (the count is not worth computing on every group)
+    cnt -= (X.bad ? 1 : 0) + (Y.bad ? 1 : 0) + (Z.bad ? 1 : 0) + (W.bad ? 1 : 0);
+  }
+}
+
+// Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
+// and staged records do not depend on where a caller splits the range.  The wavefront's C * TY
+// LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (lane l stages
+// records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for ACCUM 1
+// (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.  Groups of four
+// pulses inside every aperture of the tile, in a chunk whose reference bins all lie well inside
+// the swath, take the FAST form; both forms give the same terms.
 template <int ACCUM, bool FOURTH>
 __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                              int p1, int rec0, int C, float &sr, float &si, float &cr,
@@ -245,39 +309,44 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
   const int lane = threadIdx.x & 63;
   const float inv_dra = (float)g.inv_dra;
   const float beta_t = (float)(g.beta * INV_2PI);
-  const unsigned nbins = (unsigned)(g.nra - 1);
+  const unsigned nbin_bytes = (unsigned)(g.nra - 1) * 8u;
   const int nrec = C << s.ty_shift;
   const int rstep = 1 << s.ty_shift;
+  const int blo = s.margin, bhi = (int)(g.nra - 1) - 1 - s.margin;  // reference bins inside the swath
+  // contributors: the pulses of this lane's aperture within [c0, p1), less the rare ones whose bin
+  // falls outside the swath (counted as they occur)
+  {
+    const int lo = max(c0, t.lo), hi = min(p1, t.hi);  // compared, not subtracted: an empty aperture is
+    cnt += hi > lo ? hi - lo : 0;                       // (INT_MAX, -INT_MAX)
+  }
   for (; c0 < p1; c0 += C) {
+    bool inside = true;
     for (int m = lane; m < nrec; m += 64) {
       const int p = c0 + (m >> s.ty_shift);
-      if (p < p1) make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + m]);
+      if (p < p1) {
+        const int b0 = make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + m]);
+        inside = inside && b0 >= blo && b0 <= bhi;
+      }
     }
+    const bool swath = __builtin_amdgcn_ballot_w64(!inside) == 0;  // uniform
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row;
-    SplitHalf X, Y, Z, W;
     int k = 0;
     for (; k + 3 < cn; k += 4, rk += 4 * rstep) {
-      split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
-      split_first<FOURTH>(g, t, s, rk + rstep, c0 + k + 1, inv_dra, nbins, Y);
-      split_first<FOURTH>(g, t, s, rk + 2 * rstep, c0 + k + 2, inv_dra, nbins, Z);
-      split_first<FOURTH>(g, t, s, rk + 3 * rstep, c0 + k + 3, inv_dra, nbins, W);
-      const float2 tx = split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
-      const float2 ty = split_term(Y.go, Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
-      const float2 tz = split_term(Z.go, Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
-      const float2 tw = split_term(W.go, W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
-      // pulses c0+k .. c0+k+3 (k a multiple of 4 from the chunk start) are summed pairwise, then
-      // the group sum is accumulated with compensation
-      split_acc<ACCUM>(make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y)), sr, si, cr, ci);
-      cnt += (X.go ? 1 : 0) + (Y.go ? 1 : 0) + (Z.go ? 1 : 0) + (W.go ? 1 : 0);
+      const int a = c0 + k;
+      if (swath && a >= s.core_lo && a + 3 < s.core_hi)
+        split_group<ACCUM, FOURTH, true>(g, t, s, rk, rstep, a, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci, cnt);
+      else
+        split_group<ACCUM, FOURTH, false>(g, t, s, rk, rstep, a, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci, cnt);
     }
     for (; k < cn; ++k, rk += rstep) {  // tail (only at the end of the pulse range)
-      split_first<FOURTH>(g, t, s, rk, c0 + k, inv_dra, nbins, X);
-      split_acc<ACCUM>(split_term(X.go, X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
-      cnt += X.go ? 1 : 0;
+      SplitHalf X;
+      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_column(g, c0 + k), inv_dra, nbin_bytes, X);
+      split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
+      cnt -= X.bad ? 1 : 0;
     }
     if (ACCUM == 1) {
       ar += (double)sr;
