@@ -8,8 +8,10 @@ namespace {
 // LDS records staged per refill: C pulses x TY rows = SPLIT_SCENE_RECS (6 KiB per wavefront).
 constexpr int SPLIT_SCENE_RECS = 128;
 
+// Six wavefronts per SIMD (at most 80 VGPRs): the hot loop fits without spills; only the rare paths
+// (FP64 terms of non-compact tiles, record staging) reload a few values from scratch.
 template <int ACCUM>
-__global__ __launch_bounds__(64) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
                                                       double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
   const int C = SPLIT_SCENE_RECS / tg.ty;  // LDS: SPLIT_SCENE_RECS records (dynamic)
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);

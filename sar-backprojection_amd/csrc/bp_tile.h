@@ -69,14 +69,25 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
   return t;
 }
 
-// FP64 terms for pulses [p0, p1) of the union.
+// FP64 terms for pulses [p0, p1) of the union.  The trajectory sample is loaded one pulse ahead
+// (scalar loads), and the RawData gather is issued as soon as the bin is known, before the phase
+// and its sin/cos, which cover part of its latency.
 __device__ __forceinline__ void f64_span(const BpGeom &g, const TileLane &t, int p0, int p1, double &ar,
                                          double &ai, int32_t &cnt) {
+  if (p0 >= p1) return;
+  double4 tr = g.traj[p0];
   for (int a = p0; a < p1; ++a) {  // wave-uniform pulse
-    const double4 tr = g.traj[a];  // scalar load
-    if (a >= t.lo && a < t.hi) {
-      if (f64_term(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, g.raw + g.nra * (int64_t)a, ar, ai)) ++cnt;
+    const double4 trn = g.traj[a + 1 < p1 ? a + 1 : a];
+    const __amdgpu_buffer_rsrc_t col = raw_column(g, a);
+    double q, r2, r;
+    if (a >= t.lo && a < t.hi && f64_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r)) {
+      const float2 d = raw_load(col, (unsigned)(int)q << 3);
+      double sn, cs;
+      f64_phase(g, r, r2, sn, cs);
+      f64_acc(r2, sn, cs, d, ar, ai);
+      ++cnt;
     }
+    tr = trn;
   }
 }
 
@@ -218,12 +229,6 @@ struct SplitHalf {
 
 constexpr unsigned SPLIT_NO_SAMPLE = 0x80000000u;  // byte offset beyond every column (Nra <= 2^24)
 
-// Buffer resource of pulse a's RawData column (wave-uniform: scalar registers), bins [0, Nra-1):
-// the last bin, which never contributes (sar_focuser_puntual.py:106), reads as zero too.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_column(const BpGeom &g, int a) {
-  return __builtin_amdgcn_make_buffer_rsrc((void *)(g.raw + (int64_t)a * g.nra), (short)0,
-                                           (int)((g.nra - 1) * (int64_t)sizeof(float2)), 0x00020000);
-}
 
 // FAST: every lane with an aperture is inside it for pulse a and every bin of the chunk is inside
 // the swath (the caller checks both), so no per-lane aperture or bin-range test is needed.
@@ -261,7 +266,7 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
     h.bad = act && !ok;
     off = act && ok ? off : SPLIT_NO_SAMPLE;
   }
-  h.d = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(col, (int)off, 0, 0));
+  h.d = raw_load(col, off);
 }
 
 // Pulses are processed four at a time: the four gathers are in flight before any term is formed;

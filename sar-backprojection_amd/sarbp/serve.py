@@ -9,6 +9,7 @@ included) and aggregate pixel-pulse pairs per second.
 from __future__ import annotations
 
 import argparse
+import collections
 import ctypes
 import json
 import sys
@@ -107,9 +108,23 @@ def percentile(v, q):
 def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: float, seed: int = 0,
                 warmup_s: float = 2.0, log=None):
     """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at Poisson times of
-    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival (server clock)."""
+    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival (server clock).
+    Finished requests are harvested as the run goes (their result buffers released), so a long run
+    holds only the requests in flight."""
     rng = np.random.default_rng(seed)
-    handles = []
+    pending = collections.deque()  # (arrival offset, Handle), in submission order
+    done = []                      # (arrival offset, info)
+
+    def harvest(block):
+        while pending:
+            t_arr, h = pending[0]
+            if block:
+                h.wait()
+            elif h.state() not in (_lib.REQ_DONE, _lib.REQ_FAILED):
+                break
+            done.append((t_arr, h.info()))
+            pending.popleft()
+
     t0 = time.perf_counter()
     t_next = 0.0
     k = 0
@@ -117,22 +132,21 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
     while t_next < duration_s:
         now = time.perf_counter() - t0
         if t_next > now:
+            harvest(False)
             time.sleep(min(t_next - now, 0.05))
             continue
         di, req = pool[k % len(pool)]
-        handles.append((t_next, server.submit(datasets[di], req)))
+        pending.append((t_next, server.submit(datasets[di], req)))
         k += 1
         t_next += rng.exponential(1.0 / rate_hz)
         if log and time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
-            log(f"rate {rate_hz}/s: {k} submitted, t={now:.0f}s, {server.stats()}")
-    for _, h in handles:
-        h.wait()
+            log(f"rate {rate_hz}/s: {k} submitted, {len(pending)} in flight, t={now:.0f}s, {server.stats()}")
+    harvest(True)
     t_end = time.perf_counter() - t0
     lat, pp, prec = [], 0, {"f64": 0, "split": 0}
     first_submit, last_done = float("inf"), 0.0
-    for t_arr, h in handles:
-        inf = h.info()
+    for t_arr, inf in done:
         if t_arr >= warmup_s:
             lat.append(inf["done_s"] - inf["submit_s"])
         pp += inf["pp"]
@@ -141,7 +155,7 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
         prec["split" if inf["precision"] == _lib.PREC_SPLIT else "f64"] += 1
     span = max(1e-9, last_done - first_submit)
     return {
-        "rate_req_per_s": rate_hz, "duration_s": duration_s, "requests": len(handles),
+        "rate_req_per_s": rate_hz, "duration_s": duration_s, "requests": len(done),
         "latency_p50_ms": percentile(lat, 50) * 1e3 if lat else None,
         "latency_p99_ms": percentile(lat, 99) * 1e3 if lat else None,
         "latency_max_ms": max(lat) * 1e3 if lat else None,

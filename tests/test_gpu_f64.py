@@ -28,6 +28,7 @@ def check_parity(got, cnt, ref, rcnt, tol=REL_TOL):
     if nz.any():
         rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
         assert rel.max() <= tol, f"max rel err {rel.max():.3e} at pixel {np.nonzero(nz)[0][np.argmax(rel)]}"
+        print(f"fp64 max rel err {rel.max():.3e} over {int(nz.sum())} pixels", flush=True)
         return float(rel.max())
     return 0.0
 
