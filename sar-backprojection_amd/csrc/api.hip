@@ -57,6 +57,14 @@ int upload_traj(sarbp_dataset *ds, const double *tx, const double *ty, const dou
   return h2d(ds->traj, t.data(), t.size() * sizeof(double4), ds->stream, "trajectory upload");
 }
 
+// (sin, cos)(j*pi/128) for j = 0..255 in extended precision, rounded once to double.
+int upload_sctab(sarbp_dataset *ds) {
+  std::vector<double2> t(SARBP_SCTAB);
+  const long double h = acosl(-1.0L) / 128.0L;
+  for (int j = 0; j < SARBP_SCTAB; ++j) t[(size_t)j] = make_double2((double)sinl(h * j), (double)cosl(h * j));
+  return h2d(ds->sctab, t.data(), t.size() * sizeof(double2), ds->stream, "sin/cos table upload");
+}
+
 int dataset_init(int64_t nra, int64_t naz, const double *tx, const double *ty, const double *tz,
                  double near_range, double dra, double lambda_val, sarbp_dataset **out) {
   if (!out) return sarbp_fail(SARBP_EINVAL, "out is null");
@@ -89,6 +97,8 @@ int dataset_init(int64_t nra, int64_t naz, const double *tx, const double *ty, c
   rc = dev_alloc(&ds->raw, (size_t)nra * (size_t)naz, ds->bytes, "RawData");
   if (!rc) rc = dev_alloc(&ds->traj, (size_t)naz, ds->bytes, "trajectory");
   if (!rc) rc = upload_traj(ds, tx, ty, tz);
+  if (!rc) rc = dev_alloc(&ds->sctab, SARBP_SCTAB, ds->bytes, "sin/cos table");
+  if (!rc) rc = upload_sctab(ds);
   if (rc) {
     sarbp_dataset_destroy(ds);
     return rc;
@@ -132,6 +142,7 @@ BpGeom sarbp_dataset::geom() const {
   const double pi = SARBP_PI32_F64;
   g.kphase = -4.0 * pi / lambda_val;
   g.beta = 4.0 * pi * SARBP_ALFA_F64 / (SARBP_C_F64 * SARBP_C_F64);
+  g.sctab = sctab;
   return g;
 }
 
@@ -229,6 +240,7 @@ int sarbp_dataset_destroy(sarbp_dataset *ds) {
   if (ds->stream) (void)hipStreamSynchronize(ds->stream);
   dev_free(ds->raw);
   dev_free(ds->traj);
+  dev_free(ds->sctab);
   if (ds->ev0) (void)hipEventDestroy(ds->ev0);
   if (ds->ev1) (void)hipEventDestroy(ds->ev1);
   if (ds->stream) (void)hipStreamDestroy(ds->stream);

@@ -17,8 +17,8 @@
 
 namespace {
 
-__device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem &it, int64_t tile, int rec0,
-                                           int C) {
+__device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem &it, const double2 *tab,
+                                           int64_t tile, int rec0, int C) {
   const BpGeom g = r.g;
   const TileLane t = tile_lane(g, r.px, r.tg, tile, threadIdx.x & 63);
   const bool first = it.p_off == 0;
@@ -48,7 +48,7 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
     split = s.use;  // not a compact tile: FP64 terms (same decision every range)
   }
   if (!split) {
-    f64_span(g, t, p0, p1, ar, ai, cnt);
+    f64_span(g, tab, t, p0, p1, ar, ai, cnt);
   } else if (r.accum == 0) {
     float sr = (float)ar, si = (float)ai;  // the Kahan running sum is held exactly in the state
     split_span<0>(g, t, s, p0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
@@ -83,6 +83,8 @@ __global__ __launch_bounds__(256) void k_bp_batch(const BatchReqDev *__restrict_
                                                   const int32_t *__restrict__ seg, int32_t *heads, int C,
                                                   int max_rows) {
   __shared__ int32_t next_item;
+  __shared__ double2 tab[SARBP_SCTAB];  // every request's table holds the same values
+  load_sctab(reqs[0].g, tab);
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
   const int rec0 = wave * C * max_rows;  // this wavefront's records in sarbp_lds_rec
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(256) void k_bp_batch(const BatchReqDev *__restrict_
     const BatchItem it = items[k];
     const BatchReqDev &r = reqs[it.req];
     const int64_t tile = (int64_t)it.tile_lo + wave;
-    if (tile < r.tg.ntiles()) batch_tile(r, it, tile, rec0, C);
+    if (tile < r.tg.ntiles()) batch_tile(r, it, tab, tile, rec0, C);
     (void)waves;
   }
 }

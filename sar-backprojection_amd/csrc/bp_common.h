@@ -23,43 +23,30 @@ __device__ __forceinline__ double sqrt_rn(double x) {
   return fma(d, h, gg);
 }
 
-// sin and cos of x for |x| < 2^30: Cody-Waite reduction by pi/2 in two parts (the first
-// product and difference are exact with an FMA, so the reduced argument carries only its final
-// rounding) and Taylor polynomials on |r| <= pi/4 (+ a rounding of n): sin to r^17, cos to r^16,
-// truncation below 1e-18.  Larger |x| (never a slant-range phase) take the library sincos.
-__device__ __forceinline__ void sincos_rd(double x, double &sn, double &cs) {
-  if (!(__builtin_fabs(x) < 0x1p30)) {
+// sin and cos of x for |x| < 2^25, table-driven (P. T. P. Tang, "Table-driven implementation of
+// the logarithm / exponential function", ACM TOMS 15-16, 1989-90, applied to sin/cos): x = n h + r
+// with h = pi/128 in two parts (the first product and difference are exact with an FMA),
+// |r| <= pi/256; with (S, C) = (sin, cos)(j h), j = n mod 256, from tab:
+//   sin x = S cos r + C sin r,  cos x = C cos r - S sin r,
+// sin r to r^5 and cos r to r^6 (truncation below 1e-17 absolute).  tab may live in LDS or global
+// memory (same values, same arithmetic).  Larger |x| (never a slant-range phase: R < 80 km at
+// lambda = 3 cm) take the library sincos.
+__device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double &sn, double &cs) {
+  if (!(__builtin_fabs(x) < 0x1p25)) {
     sincos(x, &sn, &cs);
     return;
   }
-  constexpr double TWO_OVER_PI = 0.63661977236758134308;
-  constexpr double PIO2_HI = 1.5707963267948966192;     // pi/2 rounded to double
-  constexpr double PIO2_LO = 6.123233995736766036e-17;  // pi/2 - PIO2_HI
-  const double n = __builtin_rint(x * TWO_OVER_PI);
-  const double r = fma(-n, PIO2_LO, fma(-n, PIO2_HI, x));
+  constexpr double INV_H = 40.743665431525205957;            // 128/pi
+  constexpr double H_HI = 3.141592653589793116 / 128.0;      // pi rounded to double, / 128
+  constexpr double H_LO = 1.2246467991473531772e-16 / 128.0; // (pi - that) / 128
+  const double n = __builtin_rint(x * INV_H);
+  const double2 e = tab[(int)n & (SARBP_SCTAB - 1)];  // (sin, cos)(j h)
+  const double r = fma(-n, H_LO, fma(-n, H_HI, x));
   const double r2 = r * r;
-  double ps = 1.0 / 355687428096000.0;  // 1/17!
-  ps = fma(ps, r2, -1.0 / 1307674368000.0);
-  ps = fma(ps, r2, 1.0 / 6227020800.0);
-  ps = fma(ps, r2, -1.0 / 39916800.0);
-  ps = fma(ps, r2, 1.0 / 362880.0);
-  ps = fma(ps, r2, -1.0 / 5040.0);
-  ps = fma(ps, r2, 1.0 / 120.0);
-  ps = fma(ps, r2, -1.0 / 6.0);
-  const double sr = fma(r * r2, ps, r);
-  double pc = 1.0 / 20922789888000.0;  // 1/16!
-  pc = fma(pc, r2, -1.0 / 87178291200.0);
-  pc = fma(pc, r2, 1.0 / 479001600.0);
-  pc = fma(pc, r2, -1.0 / 3628800.0);
-  pc = fma(pc, r2, 1.0 / 40320.0);
-  pc = fma(pc, r2, -1.0 / 720.0);
-  pc = fma(pc, r2, 1.0 / 24.0);
-  pc = fma(pc, r2, -0.5);
-  const double cr = fma(r2, pc, 1.0);
-  const int q = (int)n & 3;  // x = r + q pi/2 (mod 2 pi)
-  const double s0 = (q & 1) ? cr : sr, c0 = (q & 1) ? sr : cr;
-  sn = (q & 2) ? -s0 : s0;
-  cs = ((q + 1) & 2) ? -c0 : c0;
+  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  sn = fma(e.x, cr, e.y * sr);
+  cs = fma(e.y, cr, -(e.x * sr));
 }
 
 // rint((r - near) / dra), the range bin of sar_focuser_puntual.py:100-101, from a product with
@@ -88,11 +75,19 @@ __device__ __forceinline__ bool f64_bin(const BpGeom &g, double x, double y, dou
   return q < (double)(g.nra - 1) && q >= 0.0;  // :106
 }
 
-// sin and cos of the phase of sar_focuser_puntual.py:116: exp(-j ph) = (cs, -sn) (:121).
-__device__ __forceinline__ void f64_phase(const BpGeom &g, double r, double r2, double &sn, double &cs) {
+// sin and cos of the phase of sar_focuser_puntual.py:116: exp(-j ph) = (cs, -sn) (:121); tab is
+// the dataset's sin/cos table (in LDS or global memory).
+__device__ __forceinline__ void f64_phase(const BpGeom &g, const double2 *tab, double r, double r2, double &sn,
+                                          double &cs) {
 #pragma clang fp contract(off)
   const double ph = g.kphase * r + g.beta * r2;
-  sincos_rd(ph, sn, cs);
+  sincos_tab(ph, tab, sn, cs);
+}
+
+// Copies the dataset's sin/cos table into the block's LDS (all threads of the block call this).
+__device__ __forceinline__ void load_sctab(const BpGeom &g, double2 *lds) {
+  for (int i = threadIdx.x; i < SARBP_SCTAB; i += blockDim.x) lds[i] = g.sctab[i];
+  __syncthreads();
 }
 
 // term = R2 * raw * exp(-j ph), added to the running sums (:144-147).
@@ -106,22 +101,27 @@ __device__ __forceinline__ void f64_acc(double r2, double sn, double cs, float2 
 
 // One pixel-pulse term against the pulse column col.  Returns false when the range bin falls
 // outside [0, nra-1).
-__device__ __forceinline__ bool f64_term(const BpGeom &g, double x, double y, double z, double tx, double ty,
-                                         double tz, const float2 *__restrict__ col, double &acc_re,
-                                         double &acc_im) {
+__device__ __forceinline__ bool f64_term(const BpGeom &g, const double2 *tab, double x, double y, double z,
+                                         double tx, double ty, double tz, const float2 *__restrict__ col,
+                                         double &acc_re, double &acc_im) {
   double q, r2, r, sn, cs;
   if (!f64_bin(g, x, y, z, tx, ty, tz, q, r2, r)) return false;
   const float2 d = col[(int)q];
-  f64_phase(g, r, r2, sn, cs);
+  f64_phase(g, tab, r, r2, sn, cs);
   f64_acc(r2, sn, cs, d, acc_re, acc_im);
   return true;
 }
 
-// Buffer resource of pulse a's RawData column (wave-uniform: scalar registers), bins [0, Nra-1):
-// the last bin, which never contributes (sar_focuser_puntual.py:106), reads as zero too.
+// Buffer resource of the RawData column starting at col (wave-uniform: scalar registers), bins
+// [0, Nra-1): the last bin, which never contributes (sar_focuser_puntual.py:106), reads as zero too.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_column_at(const float2 *col, int64_t nra) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)col, (short)0, (int)((nra - 1) * (int64_t)sizeof(float2)),
+                                           0x00020000);
+}
+
+// Buffer resource of pulse a's RawData column.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_column(const BpGeom &g, int a) {
-  return __builtin_amdgcn_make_buffer_rsrc((void *)(g.raw + (int64_t)a * g.nra), (short)0,
-                                           (int)((g.nra - 1) * (int64_t)sizeof(float2)), 0x00020000);
+  return raw_column_at(g.raw + (int64_t)a * g.nra, g.nra);
 }
 
 __device__ __forceinline__ float2 raw_load(__amdgpu_buffer_rsrc_t col, unsigned byte_off) {

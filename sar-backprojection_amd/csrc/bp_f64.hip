@@ -22,6 +22,8 @@ namespace {
 
 __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, double2 *__restrict__ out,
                                                      int32_t *__restrict__ ncnt) {
+  __shared__ double2 tab[SARBP_SCTAB];
+  load_sctab(g, tab);
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= px.n) return;
   double acc_re = 0.0, acc_im = 0.0;
@@ -33,7 +35,7 @@ __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, do
     const double x = px.x[i], y = px.y[i], z = px.z[i];
     for (int64_t a = lo; a < hi; ++a) {
       const double4 t = g.traj[a];
-      if (f64_term(g, x, y, z, t.x, t.y, t.z, g.raw + g.nra * a, acc_re, acc_im)) ++cnt;
+      if (f64_term(g, tab, x, y, z, t.x, t.y, t.z, g.raw + g.nra * a, acc_re, acc_im)) ++cnt;
     }
   }
   store_mean(out, ncnt, i, acc_re, acc_im, cnt);
@@ -42,12 +44,14 @@ __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, do
 // One wavefront per tile; block = 64 threads.
 __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, TileGrid tg, double2 *__restrict__ out,
                                                     int32_t *__restrict__ ncnt) {
+  __shared__ double2 tab[SARBP_SCTAB];
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= tg.ntiles()) return;  // whole wavefront
+  if (tile >= tg.ntiles()) return;  // whole block (one wavefront)
+  load_sctab(g, tab);
   const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
   double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
-  f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
+  f64_span(g, tab, t, t.wlo, t.whi, ar, ai, cnt);
   if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 

@@ -22,7 +22,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   if (t.wlo < t.whi) {
     const SplitTile s = split_tile(g, tg, t);
     if (!s.use) {  // not a compact tile: FP64 terms
-      f64_span(g, t, t.wlo, t.whi, ar, ai, cnt);
+      f64_span(g, g.sctab, t, t.wlo, t.whi, ar, ai, cnt);  // table read from global memory (rare path)
     } else {
       float sr = 0.f, si = 0.f, cr = 0.f, ci = 0.f;
       split_span<ACCUM>(g, t, s, t.wlo, t.whi, 0, C, sr, si, cr, ci, ar, ai, cnt);

@@ -69,11 +69,11 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
   return t;
 }
 
-// FP64 terms for pulses [p0, p1) of the union.  The trajectory sample is loaded one pulse ahead
+// FP64 terms for pulses [p0, p1) of the union (tab: the sin/cos table).  The trajectory sample is loaded one pulse ahead
 // (scalar loads), and the RawData gather is issued as soon as the bin is known, before the phase
 // and its sin/cos, which cover part of its latency.
-__device__ __forceinline__ void f64_span(const BpGeom &g, const TileLane &t, int p0, int p1, double &ar,
-                                         double &ai, int32_t &cnt) {
+__device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0, int p1,
+                                         double &ar, double &ai, int32_t &cnt) {
   if (p0 >= p1) return;
   double4 tr = g.traj[p0];
   for (int a = p0; a < p1; ++a) {  // wave-uniform pulse
@@ -83,7 +83,7 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const TileLane &t, int
     if (a >= t.lo && a < t.hi && f64_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r)) {
       const float2 d = raw_load(col, (unsigned)(int)q << 3);
       double sn, cs;
-      f64_phase(g, r, r2, sn, cs);
+      f64_phase(g, tab, r, r2, sn, cs);
       f64_acc(r2, sn, cs, d, ar, ai);
       ++cnt;
     }
@@ -276,11 +276,12 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
 // result -- does not depend on how a caller splits the pulse range.
 template <int ACCUM, bool FOURTH, bool FAST>
 __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                            const PulseRec *rk, int rstep, int a, float inv_dra,
-                                            unsigned nbin_bytes, float beta_t, float &sr, float &si, float &cr,
-                                            float &ci, int32_t &cnt) {
-  const __amdgpu_buffer_rsrc_t c0r = raw_column(g, a), c1r = raw_column(g, a + 1), c2r = raw_column(g, a + 2),
-                               c3r = raw_column(g, a + 3);
+                                            const PulseRec *rk, int rstep, int a, const float2 *col,
+                                            float inv_dra, unsigned nbin_bytes, float beta_t, float &sr, float &si,
+                                            float &cr, float &ci, int32_t &cnt) {
+  const int64_t nra = g.nra;  // col: the column of pulse a
+  const __amdgpu_buffer_rsrc_t c0r = raw_column_at(col, nra), c1r = raw_column_at(col + nra, nra),
+                               c2r = raw_column_at(col + 2 * nra, nra), c3r = raw_column_at(col + 3 * nra, nra);
   SplitHalf X, Y, Z, W;
   split_first<FOURTH, FAST>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
   split_first<FOURTH, FAST>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
@@ -339,17 +340,20 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row;
+    const float2 *col = g.raw + (int64_t)c0 * g.nra;  // column of pulse c0 + k
     int k = 0;
-    for (; k + 3 < cn; k += 4, rk += 4 * rstep) {
+    for (; k + 3 < cn; k += 4, rk += 4 * rstep, col += 4 * g.nra) {
       const int a = c0 + k;
       if (swath && a >= s.core_lo && a + 3 < s.core_hi)
-        split_group<ACCUM, FOURTH, true>(g, t, s, rk, rstep, a, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci, cnt);
+        split_group<ACCUM, FOURTH, true>(g, t, s, rk, rstep, a, col, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci,
+                                         cnt);
       else
-        split_group<ACCUM, FOURTH, false>(g, t, s, rk, rstep, a, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci, cnt);
+        split_group<ACCUM, FOURTH, false>(g, t, s, rk, rstep, a, col, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci,
+                                          cnt);
     }
-    for (; k < cn; ++k, rk += rstep) {  // tail (only at the end of the pulse range)
+    for (; k < cn; ++k, rk += rstep, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
-      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_column(g, c0 + k), inv_dra, nbin_bytes, X);
+      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_column_at(col, g.nra), inv_dra, nbin_bytes, X);
       split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt -= X.bad ? 1 : 0;
     }

@@ -26,7 +26,11 @@ struct BpGeom {
   double inv_dra;       // 1/dra, used only where the result is re-checked exactly
   double kphase;        // (-4*PI32)/lambda        (sar_focuser_puntual.py:116)
   double beta;          // ((4*PI32)*ALFA)/(C*C)   (sar_focuser_puntual.py:40)
+  const double2 *sctab; // SARBP_SCTAB entries (sin, cos)(j*pi/128), j = 0..255, for sincos_tab
 };
+
+// Entries of the sin/cos table of the FP64 path (one turn in steps of pi/128).
+#define SARBP_SCTAB 256
 
 // Pixel set in device memory (F-order flattened DEM grid).
 struct PixelsView {
@@ -47,6 +51,7 @@ struct sarbp_dataset {
   double near_range = 0, dra = 0, lambda_val = 0;
   float2 *raw = nullptr;
   double4 *traj = nullptr;
+  double2 *sctab = nullptr;           // SARBP_SCTAB (sin, cos) pairs
   std::vector<double> htx, hty, htz;  // host copy of the trajectory (request planning)
   int64_t bytes = 0;
   BpGeom geom() const;
