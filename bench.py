@@ -95,6 +95,40 @@ def cpu_baseline(sc, strip_cols, serial_cols, gpu_strip):
                   "strip_zero_pixels_exact": zeros_ok, "strip_pixels_checked": int(nz.sum())}
 
 
+def small_scene_leg(cores):
+    """The reference's CPU-runnable case (500 x 50 px, 5001 pulses, Sint 400): GPU FP64 and split
+    rates and parity against the oracle, and the oracle itself on all cores and on one thread."""
+    from oracle import oracle_lib as O
+    from sarbp import synth
+    from sarbp.core import Dataset
+    sc = synth.small_scene(seed=0)
+    ds = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    scene = ds.scene(*sc.flat(), dimx=sc.dimx)
+    pp = scene.count_pp()
+    out = {"workload": "500x50 px x 5001 pulses, Nra 600, Sint 400", "pp": int(pp)}
+    res = {}
+    for prec in ("f64", "split"):
+        _, ks = time_precision(scene, prec, 5, 1)
+        out[f"gpu_{prec}_pp_per_s"] = pp / ks
+        res[prec] = scene.result(with_counts=True)
+    scene.close()
+    ds.close()
+    t0 = time.perf_counter()
+    ref, rcnt = O.focus(*sc.args(), nthreads=cores)
+    out["cpu_pp_per_s"] = pp / (time.perf_counter() - t0)
+    ref, rcnt = ref.ravel(order="F"), rcnt.ravel(order="F")  # the scene's flat (F-order) pixel order
+    out["cpu_cores"] = int(cores)
+    t0 = time.perf_counter()
+    O.focus(*sc.args(), nthreads=1)
+    out["cpu_serial_pp_per_s"] = pp / (time.perf_counter() - t0)
+    got, cnt = res["f64"]
+    nz = rcnt > 0
+    out["f64_counts_exact"] = bool(np.array_equal(cnt, rcnt))
+    out["f64_max_rel_err"] = float((np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])).max())
+    out["split_vs_oracle"] = phase_amp_errors(res["split"][0], ref)
+    return out
+
+
 def target_peaks(sc, img_split, img_f64, half_x=3, half_y=2):
     """Point-target responses: for each simulated target the peak of |image| in a
     window around its pixel, split vs FP64: same peak pixel, relative peak-height difference."""
@@ -246,6 +280,8 @@ def main():
 
         cpu, parity = cpu_baseline(sc, args.strip, args.serial_strip, gpu_strip)
         extra.update(parity)
+        extra["small_scene"] = small_scene_leg(cpu["cores"])
+        log(f"small scene: {extra['small_scene']}")
         log(f"cpu baseline {cpu['value']:.4e} pp/s on {cpu['cores']} cores; strip parity {parity}")
 
     r = res[head]
