@@ -203,7 +203,8 @@ typedef struct sarbp_server_stats {
 int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
 /* Copy the request's inputs and queue it.  out_c128 (2*n float64) and ncontrib (n int32,
  * nullable) receive the result when the request completes and must stay valid until then.
- * SARBP_EBUDGET if the request alone exceeds the budget. */
+ * SARBP_EBUDGET if the request alone exceeds the budget.  Once a request is done or failed the
+ * server keeps only its record (state, timings, counts): its copy of the inputs is released. */
 int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
                         int32_t *ncontrib, int64_t *id);
 int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state);

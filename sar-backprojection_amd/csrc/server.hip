@@ -119,6 +119,22 @@ struct sarbp_server {
     r->on_device = false;
     bytes_in_use -= r->need;
   }
+  // A finished (done or failed) request keeps only its record: its host copy of the pixel arrays
+  // and any saved state are dropped, so a long-running server holds the pixels of the requests in
+  // flight only.
+  static void finish(ServerReq *r, int32_t state) {
+    r->state = state;
+    std::vector<double>().swap(r->x);
+    std::vector<double>().swap(r->y);
+    std::vector<double>().swap(r->z);
+    std::vector<uint8_t>().swap(r->mask);
+    std::vector<int32_t>().swap(r->index);
+    std::vector<int32_t>().swap(r->sint);
+    std::vector<BpState>().swap(r->saved);
+    r->req.x = r->req.y = r->req.z = nullptr;
+    r->req.mask = nullptr;
+    r->req.index = r->req.sint = nullptr;
+  }
 };
 
 bool sarbp_server::admit(ServerReq *r) {
@@ -161,7 +177,7 @@ void sarbp_server::run() {
       lk.lock();
       if (!ok) {
         bytes_in_use -= r->need;
-        r->state = SARBP_REQ_FAILED;
+        finish(r, SARBP_REQ_FAILED);
         stats.failed++;
         cv_done.notify_all();
         continue;
@@ -186,8 +202,12 @@ void sarbp_server::run() {
       lk.lock();
       release(r);
       r->preempt = false;
-      r->state = e == hipSuccess ? SARBP_REQ_PREEMPTED : SARBP_REQ_FAILED;
-      if (e != hipSuccess) stats.failed++;
+      if (e == hipSuccess) {
+        r->state = SARBP_REQ_PREEMPTED;
+      } else {
+        finish(r, SARBP_REQ_FAILED);
+        stats.failed++;
+      }
       r->preemptions++;
       it = active.erase(it);
       cv_done.notify_all();
@@ -245,7 +265,7 @@ void sarbp_server::run() {
       r->rounds++;
       if (failed) {
         release(r);
-        r->state = SARBP_REQ_FAILED;
+        finish(r, SARBP_REQ_FAILED);
         stats.failed++;
         continue;
       }
@@ -262,11 +282,11 @@ void sarbp_server::run() {
         release(r);
         r->done_s = now();
         if (ce == hipSuccess) {
-          r->state = SARBP_REQ_DONE;
+          finish(r, SARBP_REQ_DONE);
           stats.completed++;
           stats.pp_done += r->pp;
         } else {
-          r->state = SARBP_REQ_FAILED;
+          finish(r, SARBP_REQ_FAILED);
           stats.failed++;
         }
       }
@@ -279,10 +299,10 @@ void sarbp_server::run() {
   // shutdown: fail what is left
   for (ServerReq *r : active) {
     release(r);
-    r->state = SARBP_REQ_FAILED;
+    finish(r, SARBP_REQ_FAILED);
   }
   active.clear();
-  for (ServerReq *r : queue) r->state = SARBP_REQ_FAILED;
+  for (ServerReq *r : queue) finish(r, SARBP_REQ_FAILED);
   queue.clear();
   cv_done.notify_all();
 }
