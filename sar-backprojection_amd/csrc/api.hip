@@ -315,6 +315,7 @@ int sarbp_scene_focus(sarbp_dataset *ds, sarbp_scene *sc, int precision, float *
   if (sc->ds != ds) return sarbp_fail(SARBP_EINVAL, "scene belongs to another dataset");
   if (precision != SARBP_PREC_F64 && precision != SARBP_PREC_SPLIT)
     return sarbp_fail(SARBP_EINVAL, "unknown precision");
+  std::lock_guard<std::mutex> lk(ds->timing_mu);  // scenes of one dataset may be focused from several threads
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
   int rc = precision == SARBP_PREC_SPLIT ? sarbp_launch_split(ds, sc, sc->tiling_split, ds->stream)

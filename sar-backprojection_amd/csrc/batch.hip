@@ -218,6 +218,7 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   int rc = batch_build(b, waves);
   if (rc) return rc;
   sarbp_dataset *ds = b->ds;
+  std::lock_guard<std::mutex> lk(ds->timing_mu);
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
   if (b->ib.nitems > 0) {

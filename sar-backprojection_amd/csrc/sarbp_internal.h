@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,7 @@ struct sarbp_dataset {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::mutex timing_mu;  // the timed launches of the dataset's scenes and batches share ev0/ev1
   int64_t nra = 0, naz = 0;
   double near_range = 0, dra = 0, lambda_val = 0;
   float2 *raw = nullptr;
