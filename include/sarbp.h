@@ -151,6 +151,17 @@ int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, in
 int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t *ncontrib);
 int sarbp_batch_destroy(sarbp_batch *b);
 
+/*
+ * Diagnostic: the FP64 path's arithmetic for one pixel-pulse pair (sar_focuser_puntual.py:82-121)
+ * evaluated on the device for n pairs: r2 = (dx^2 + dy^2) + dz^2, r = sqrt(r2), bin =
+ * rint((r - near)/dra) half-even (-1 when outside [0, nra-1)), ph = k r + beta r2 and (sin, cos)(ph),
+ * with x, y, z float64[n] pixel coordinates and pulse int32[n] in [0, naz).  Outputs (each [n])
+ * may be NULL.  No counterpart in the reference: it lets a caller verify the device arithmetic
+ * against a CPU evaluation bit for bit (r2, r, bin, ph) and to the last ulp (sin, cos).
+ */
+int sarbp_eval_pairs(sarbp_dataset *ds, const double *x, const double *y, const double *z, const int32_t *pulse,
+                     int64_t n, double *r2, double *r, int32_t *bin, double *ph, double *sn, double *cs);
+
 /* ---- serving: continuous batching of requests against resident datasets -------------------- */
 
 /*

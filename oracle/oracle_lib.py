@@ -42,6 +42,9 @@ def lib():
         L.sarbp_oracle_trace.argtypes = [_d, _d, _d, _i32, _i32, _p, _p, _p, _i64, _i64, _d, _d, _p,
                                          _p, _p, _i64]
         L.sarbp_oracle_trace.restype = _i64
+        L.sarbp_oracle_pairs.argtypes = [_p, _p, _p, _p, _i64, _p, _p, _p, _i64, _d, _d, _d] + [_p] * 6 + \
+            [ctypes.c_int]
+        L.sarbp_oracle_pairs.restype = None
         L.sarbp_oracle_count_pp.argtypes = [_p, _p, _p, _i64, _i64]
         L.sarbp_oracle_count_pp.restype = _i64
         _lib = L
@@ -101,3 +104,19 @@ def count_pp(mask, index, sint, Naz) -> int:
     mask, index, sint = _c(mask, np.uint8).ravel(), _c(index, np.int32).ravel(), _c(sint, np.int32).ravel()
     return int(lib().sarbp_oracle_count_pp(mask.ctypes.data, index.ctypes.data, sint.ctypes.data,
                                            mask.size, Naz))
+
+
+def pairs(x, y, z, pulse, tx, ty, tz, Nra, near, dra, lam, nthreads=0):
+    """Per-(pixel, pulse) R2, R, bin (-1 outside [0, Nra-1)), phase, sin, cos of
+    sar_focuser_puntual.py:82-121 for equal-length arrays x, y, z (float64) and pulse (int32)."""
+    x, y, z = _c(x, np.float64), _c(y, np.float64), _c(z, np.float64)
+    pulse = _c(pulse, np.int32)
+    tx, ty, tz = _c(tx, np.float64), _c(ty, np.float64), _c(tz, np.float64)
+    n = x.size
+    out = {k: np.empty(n, np.float64) for k in ("r2", "r", "ph", "sin", "cos")}
+    out["bin"] = np.empty(n, np.int32)
+    lib().sarbp_oracle_pairs(x.ctypes.data, y.ctypes.data, z.ctypes.data, pulse.ctypes.data, n, tx.ctypes.data,
+                             ty.ctypes.data, tz.ctypes.data, int(Nra), float(near), float(dra), float(lam),
+                             out["r2"].ctypes.data, out["r"].ctypes.data, out["bin"].ctypes.data,
+                             out["ph"].ctypes.data, out["sin"].ctypes.data, out["cos"].ctypes.data, int(nthreads))
+    return out

@@ -93,6 +93,40 @@ int32_t sarbp_oracle_point(double x, double y, double z, uint8_t mask, int32_t i
     return n;
 }
 
+/* The per-(pixel, pulse) arithmetic of process_point (sar_focuser_puntual.py:82-121) for n pairs:
+ * R2, R, the range bin (-1 when outside [0, Nra-1)), the phase and (sin, cos) of it.  Outputs
+ * may be NULL.  Used to check the device arithmetic bit for bit. */
+void sarbp_oracle_pairs(const double *x, const double *y, const double *z, const int32_t *pulse, int64_t n,
+                        const double *tx, const double *ty, const double *tz, int64_t Nra,
+                        double near_range, double dra, double lambda_val,
+                        double *r2o, double *ro, int32_t *bino, double *pho, double *sno, double *cso,
+                        int nthreads)
+{
+    const double beta = 4.0 * ORC_PI * ORC_ALFA / (ORC_C * ORC_C); /* sfp:40 */
+    const double kphase = -4.0 * ORC_PI / lambda_val;               /* sfp:116 */
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nt)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t a = pulse[i];
+        double dx = x[i] - tx[a];
+        double dy = y[i] - ty[a];
+        double dz = z[i] - tz[a];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        double r = sqrt(r2);
+        double q = nearbyint((r - near_range) / dra);
+        double ph = kphase * r + beta * r2;
+        if (r2o) r2o[i] = r2;
+        if (ro) ro[i] = r;
+        if (bino) bino[i] = (q < (double)(Nra - 1) && q >= 0.0) ? (int32_t)q : -1;
+        if (pho) pho[i] = ph;
+        if (sno) sno[i] = sin(ph);
+        if (cso) cso[i] = cos(ph);
+    }
+    (void)nthreads;
+}
+
 /* Pixel loop of FocalizzatoreBpPx (sar_focuser_puntual.py:208-211, 259-268) over F-order
  * flattened pixels.  out is interleaved complex128 [n]; ncontrib may be NULL.
  * nthreads <= 0 uses the OpenMP default; 1 reproduces the reference's serial loop. */

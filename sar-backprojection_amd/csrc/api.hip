@@ -248,6 +248,51 @@ int sarbp_dataset_destroy(sarbp_dataset *ds) {
   return SARBP_OK;
 }
 
+int sarbp_eval_pairs(sarbp_dataset *ds, const double *x, const double *y, const double *z, const int32_t *pulse,
+                     int64_t n, double *r2, double *r, int32_t *bin, double *ph, double *sn, double *cs) {
+  if (!ds) return sarbp_fail(SARBP_EINVAL, "null dataset");
+  if (n < 0 || (n > 0 && (!x || !y || !z || !pulse))) return sarbp_fail(SARBP_EINVAL, "bad pair arrays");
+  for (int64_t i = 0; i < n; ++i)
+    if (pulse[i] < 0 || pulse[i] >= ds->naz) return sarbp_fail(SARBP_EINVAL, "pulse index outside [0, naz)");
+  if (n == 0) return SARBP_OK;
+  const size_t un = (size_t)n;
+  char *buf = nullptr;
+  const size_t in_bytes = un * (3 * sizeof(double) + sizeof(int32_t));
+  const size_t out_bytes = un * (5 * sizeof(double) + sizeof(int32_t));
+  hipError_t e = hipMalloc((void **)&buf, in_bytes + out_bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return sarbp_fail(SARBP_ENOMEM, "pair buffers");
+  }
+  double *dx = (double *)buf, *dy = dx + un, *dz = dy + un;
+  double *o_r2 = dz + un, *o_r = o_r2 + un, *o_ph = o_r + un, *o_sn = o_ph + un, *o_cs = o_sn + un;
+  int32_t *dp = (int32_t *)(o_cs + un), *o_bin = dp + un;
+  int rc = h2d(dx, x, un * 8, ds->stream, "pair upload");
+  if (!rc) rc = h2d(dy, y, un * 8, ds->stream, "pair upload");
+  if (!rc) rc = h2d(dz, z, un * 8, ds->stream, "pair upload");
+  if (!rc) rc = h2d(dp, pulse, un * 4, ds->stream, "pair upload");
+  if (!rc) rc = sarbp_launch_eval_pairs(ds, dx, dy, dz, dp, n, o_r2, o_r, o_bin, o_ph, o_sn, o_cs, ds->stream);
+  struct Out {
+    void *host;
+    const void *dev;
+    size_t bytes;
+  } outs[6] = {{r2, o_r2, un * 8}, {r, o_r, un * 8}, {bin, o_bin, un * 4}, {ph, o_ph, un * 8}, {sn, o_sn, un * 8},
+               {cs, o_cs, un * 8}};
+  for (const Out &o : outs) {
+    if (rc || !o.host) continue;
+    e = hipMemcpyAsync(o.host, o.dev, o.bytes, hipMemcpyDeviceToHost, ds->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ds->stream);
+    if (e != hipSuccess) rc = sarbp_hip_fail(e, "pair download");
+  }
+  if (!rc) {
+    e = hipStreamSynchronize(ds->stream);
+    if (e != hipSuccess) rc = sarbp_hip_fail(e, "pair kernel");
+  }
+  (void)hipStreamSynchronize(ds->stream);
+  (void)hipFree(buf);
+  return rc;
+}
+
 int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes) {
   if (!ds || !bytes) return sarbp_fail(SARBP_EINVAL, "null argument");
   *bytes = ds->bytes;

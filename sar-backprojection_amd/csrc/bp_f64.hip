@@ -55,7 +55,48 @@ __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, Til
   if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 
+// The per-pair arithmetic of the FP64 path (f64_bin, f64_phase) for n (pixel, pulse) pairs, for
+// verification against the oracle: R2, R, bin (-1 outside [0, nra-1)), phase, sin, cos.
+__global__ __launch_bounds__(256) void k_eval_pairs(BpGeom g, const double *__restrict__ x, const double *__restrict__ y,
+                                                    const double *__restrict__ z, const int32_t *__restrict__ pulse,
+                                                    int64_t n, double *__restrict__ r2o, double *__restrict__ ro,
+                                                    int32_t *__restrict__ bino, double *__restrict__ pho,
+                                                    double *__restrict__ sno, double *__restrict__ cso) {
+  __shared__ double2 tab[SARBP_SCTAB];
+  load_sctab(g, tab);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 t = g.traj[pulse[i]];  // pulse range checked on the host
+  double q, r2, r, sn, cs;
+  const bool in = f64_bin(g, x[i], y[i], z[i], t.x, t.y, t.z, q, r2, r);
+  double ph;
+  {
+#pragma clang fp contract(off)
+    ph = g.kphase * r + g.beta * r2;
+  }
+  f64_phase(g, tab, r, r2, sn, cs);
+  r2o[i] = r2;
+  ro[i] = r;
+  bino[i] = in ? (int32_t)q : -1;
+  pho[i] = ph;
+  sno[i] = sn;
+  cso[i] = cs;
+}
+
 }  // namespace
+
+int sarbp_launch_eval_pairs(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                            const int32_t *pulse, int64_t n, double *r2, double *r, int32_t *bin, double *ph,
+                            double *sn, double *cs, hipStream_t st) {
+  if (n <= 0) return SARBP_OK;
+  const int64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many pairs for one launch");
+  hipLaunchKernelGGL(k_eval_pairs, dim3((unsigned)blocks), dim3(256), 0, st, ds->geom(), x, y, z, pulse, n, r2, r, bin,
+                     ph, sn, cs);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return sarbp_hip_fail(e, "pair kernel launch");
+  return SARBP_OK;
+}
 
 int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st) {
   if (sc->n == 0) return SARBP_OK;

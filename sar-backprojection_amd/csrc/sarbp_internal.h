@@ -114,6 +114,9 @@ int sarbp_hip_fail(hipError_t e, const char *what);
 // kernel launchers (bp_f64.hip, bp_split.hip)
 int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st);
 int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Tiling &t, hipStream_t st);
+int sarbp_launch_eval_pairs(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                            const int32_t *pulse, int64_t n, double *r2, double *r, int32_t *bin, double *ph,
+                            double *sn, double *cs, hipStream_t st);
 
 // Grid tile -> lanes.  Tiles are numbered x-fastest inside a band of tile rows; blocks are remapped
 // so that each XCD (blocks are dealt round-robin to the 8 XCDs) works on a contiguous run of tiles

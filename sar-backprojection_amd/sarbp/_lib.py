@@ -83,6 +83,7 @@ SIGNATURES = {
     "sarbp_scene_download": (ctypes.c_int, [_p, _p, _p]),
     "sarbp_focus": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, ctypes.c_int, _p, _p]),
     "sarbp_count_pp": (_i64, [_p, _p, _p, _i64, _i64]),
+    "sarbp_eval_pairs": (ctypes.c_int, [_p, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p]),
     "sarbp_batch_create": (ctypes.c_int, [_p, _RP, _i32, _pp]),
     "sarbp_batch_run": (ctypes.c_int, [_p, _i32, _i32, _fp]),
     "sarbp_batch_autotune": (ctypes.c_int, [_p, _i32, _i32p, _i32p, _fp]),

@@ -140,6 +140,22 @@ class Dataset:
                                       out.ctypes.data, cnt.ctypes.data), "sarbp_focus")
         return (out, cnt) if with_counts else out
 
+    def eval_pairs(self, x, y, z, pulse) -> dict:
+        """The FP64 path's per-(pixel, pulse) arithmetic on the device (sarbp_eval_pairs): R2, R,
+        bin (-1 outside [0, Nra-1)), phase, sin, cos for equal-length x, y, z and pulse arrays."""
+        x, y, z = _c(x, np.float64).ravel(), _c(y, np.float64).ravel(), _c(z, np.float64).ravel()
+        pulse = _c(pulse, np.int32).ravel()
+        n = x.size
+        if not (y.size == z.size == pulse.size == n):
+            raise ValueError("pair arrays must all have the same size")
+        out = {k: np.empty(n, np.float64) for k in ("r2", "r", "ph", "sin", "cos")}
+        out["bin"] = np.empty(n, np.int32)
+        check(_lib.load().sarbp_eval_pairs(self._h, x.ctypes.data, y.ctypes.data, z.ctypes.data, pulse.ctypes.data, n,
+                                           out["r2"].ctypes.data, out["r"].ctypes.data, out["bin"].ctypes.data,
+                                           out["ph"].ctypes.data, out["sin"].ctypes.data, out["cos"].ctypes.data),
+              "sarbp_eval_pairs")
+        return out
+
 
 class Scene:
     _close_order = 1  # servers, then batches/scenes, then datasets

@@ -168,3 +168,59 @@ def test_resume_from_existing_checkpoint(gpu, tmp_path):
     # a checkpoint of another size is ignored (fresh start)
     formats.save_px_checkpoint("focalizador_px_checkpoint.npz", np.ones(5, np.complex128), 3)
     assert np.array_equal(FocalizzatoreBpPx(*sc.args()), full)
+
+
+def test_pair_arithmetic_bitwise_1e8_pairs(gpu):
+    # R2, R, the range bin and the phase of 1e8 random (pixel, pulse) pairs at the full workload's
+    # magnitudes (R 4.2-6 km, phase ~2e6 rad) equal the oracle's bit for bit; sin and cos agree to
+    # the last ulp or two (device table-driven sincos vs glibc)
+    rng = np.random.default_rng(11)
+    naz, nra, near, dra, lam = 4001, 4762, 4000.0, 0.5, 0.031
+    tx = np.arange(naz) * 0.1 + rng.normal(0, 0.05, naz)
+    ty = -3000.0 + rng.normal(0, 0.05, naz)
+    tz = 3000.0 + rng.normal(0, 0.05, naz)
+    raw = np.zeros((nra, naz), np.complex64, order="F")
+    ds = Dataset(raw, tx, ty, tz, near, dra, lam)
+    try:
+        total, nb = 0, 0
+        for batch in range(10):
+            n = 10_000_000
+            print(f"case pairs batch {batch} n={n}", flush=True)
+            x = rng.uniform(-1500.0, 1900.0, n)
+            y = rng.uniform(0.0, 1000.0, n)
+            z = rng.uniform(0.0, 50.0, n)
+            pulse = rng.integers(0, naz, n).astype(np.int32)
+            got = ds.eval_pairs(x, y, z, pulse)
+            ref = O.pairs(x, y, z, pulse, tx, ty, tz, nra, near, dra, lam)
+            for k in ("r2", "r", "ph"):
+                assert np.array_equal(got[k], ref[k]), (k, np.count_nonzero(got[k] != ref[k]))
+            assert np.array_equal(got["bin"], ref["bin"])
+            for k in ("sin", "cos"):
+                err = np.abs(got[k] - ref[k]).max()
+                assert err <= 4e-16, (k, err)
+            total += n
+            nb += int(np.count_nonzero(ref["bin"] >= 0))
+        assert total == 100_000_000 and nb > 0.5 * total
+    finally:
+        ds.close()
+
+
+def test_golden_fixture(gpu):
+    # the committed 16 x 8 fixture (tests/golden/make_golden.py), FP64 and split paths
+    from test_oracle import _golden
+    g = _golden()
+    print("case golden 16x8", flush=True)
+    f = lambda a: np.asarray(a).flatten(order="F")  # noqa: E731
+    ds = Dataset(g["raw"], g["tx"], g["ty"], g["tz"], float(g["near"]), float(g["dra"]), float(g["lam"]))
+    try:
+        args = [f(g[k]) for k in ("x", "y", "z", "mask", "index", "sint")]
+        ref, rcnt = f(g["out"]), f(g["ncontrib"])
+        got, cnt = ds.focus_flat(*args, precision="f64", with_counts=True, dimx=g["x"].shape[0])
+        check_parity(got, cnt, ref, rcnt)
+        got, cnt = ds.focus_flat(*args, precision="split", with_counts=True, dimx=g["x"].shape[0])
+        assert np.array_equal(cnt, rcnt)
+        nz = rcnt > 0
+        d = np.angle(got[nz] / ref[nz])
+        assert np.sqrt(np.mean(d ** 2)) <= 1e-3 and np.all(got[~nz] == 0)
+    finally:
+        ds.close()
