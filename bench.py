@@ -234,6 +234,10 @@ def main():
     for p, r in res.items():
         extra[f"{p}_pp_per_s"] = r["pp_per_s"]
         extra[f"{p}_kernel_ms"] = r["kernel_s"] * 1e3
+        # the same gather-bandwidth roofline for each path, and the FP64-VALU one (26 flop/pp
+        # against 78.6 TFLOP/s) for reference
+        extra[f"{p}_roofline_frac"] = BYTES_PER_PP * pp / r["kernel_s"] / 1e9 / HBM_PEAK_GBS
+        extra[f"{p}_fp64_valu_frac"] = 26.0 * pp / r["kernel_s"] / 78.6e12
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
         extra["split_vs_f64_errors"] = phase_amp_errors(outs["split"], outs["f64"])
