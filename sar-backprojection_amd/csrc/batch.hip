@@ -19,8 +19,76 @@ std::mutex g_tune_mu;
 std::map<std::tuple<int64_t, int64_t, int64_t, int32_t>, std::pair<int, int>> g_tuned;  // -> (tile_px, chunk)
 }  // namespace
 
+char *DevicePool::get(size_t bytes, size_t *cap) {
+  const size_t want = (bytes + ((size_t)1 << 20) - 1) >> 20 << 20;
+  auto it = blocks.lower_bound(want);
+  if (it != blocks.end() && it->first <= 2 * want) {
+    char *p = it->second;
+    *cap = it->first;
+    cached -= it->first;
+    blocks.erase(it);
+    return p;
+  }
+  char *p = nullptr;
+  hipError_t e = hipMalloc((void **)&p, want);
+  if (e != hipSuccess && !blocks.empty()) {  // give the cached blocks back and retry once
+    (void)hipGetLastError();
+    release();
+    e = hipMalloc((void **)&p, want);
+  }
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  *cap = want;
+  return p;
+}
+
+void DevicePool::put(char *p, size_t cap) {
+  if (!p) return;
+  if (cached + cap > limit) {
+    (void)hipFree(p);
+    return;
+  }
+  blocks.emplace(cap, p);
+  cached += cap;
+}
+
+void DevicePool::release() {
+  for (auto &b : blocks) (void)hipFree(b.second);
+  blocks.clear();
+  cached = 0;
+}
+
+void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
+                   int64_t &pp) {
+  const int tx = r.tile_x ? r.tile_x : default_tiling(r.precision).tile_x;
+  const int64_t dimx = r.dimx > 0 ? r.dimx : (r.n > 0 ? r.n : 1);
+  const int64_t dimy = r.n > 0 ? r.n / dimx : 0;
+  tg = make_tile_grid(dimx, dimy, tx);
+  const int64_t nt = tg.ntiles();
+  tile_len.assign((size_t)nt, 0);
+  pp = 0;
+  if (r.n <= 0) return;
+  std::vector<int> tlo((size_t)nt, INT_MAX), thi((size_t)nt, INT_MIN);
+  for (int64_t iy = 0; iy < dimy; ++iy)
+    for (int64_t ix = 0; ix < dimx; ++ix) {
+      const int64_t i = ix + dimx * iy;
+      if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
+      int64_t lo, hi;
+      sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, lo, hi);
+      if (hi <= lo) continue;
+      pp += hi - lo;
+      const int64_t t = (iy / tg.ty) * tg.ntx + ix / tg.tx;
+      tlo[(size_t)t] = std::min(tlo[(size_t)t], (int)lo);
+      thi[(size_t)t] = std::max(thi[(size_t)t], (int)hi);
+    }
+  for (int64_t t = 0; t < nt; ++t)
+    tile_len[(size_t)t] = thi[(size_t)t] > tlo[(size_t)t] ? thi[(size_t)t] - tlo[(size_t)t] : 0;
+}
+
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b,
-                  hipStream_t st) {
+                  hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
   if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
@@ -48,12 +116,18 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
   off[9] = o;
   b.bytes = (int64_t)o;
   if (o > 0) {
-    hipError_t e = hipMalloc((void **)&b.base, o);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      b.base = nullptr;
-      return sarbp_fail(SARBP_ENOMEM, "request device allocation failed");
+    b.pool = pool;
+    if (pool) {
+      b.base = pool->get(o, &b.cap);
+    } else {
+      hipError_t e = hipMalloc((void **)&b.base, o);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        b.base = nullptr;
+      }
+      b.cap = o;
     }
+    if (!b.base) return sarbp_fail(SARBP_ENOMEM, "request device allocation failed");
   }
   char *p = b.base;
   b.x = (double *)(p + off[0]);
@@ -70,6 +144,7 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
   for (int k = 0; k < 6 && n; ++k) {
     hipError_t e = hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, st);
     if (e != hipSuccess) {
+      (void)hipStreamSynchronize(st);  // nothing in flight may still write the block
       reqbuf_free(b);
       return sarbp_hip_fail(e, "request upload");
     }
@@ -78,35 +153,28 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
     hipError_t e = hipMemsetAsync(b.out, 0, n * 16, st);
     if (e == hipSuccess) e = hipMemsetAsync(b.ncnt, 0, n * 4, st);
     if (e != hipSuccess) {
+      (void)hipStreamSynchronize(st);
       reqbuf_free(b);
       return sarbp_hip_fail(e, "request output reset");
     }
   }
   // per-tile union lengths (host, integers only): work estimate for scheduling
-  const int64_t nt = b.tg.ntiles();
-  b.tile_len.assign((size_t)nt, 0);
-  if (n) {
-    std::vector<int> tlo((size_t)nt, INT_MAX), thi((size_t)nt, INT_MIN);
-    for (int64_t iy = 0; iy < b.dimy; ++iy)
-      for (int64_t ix = 0; ix < b.dimx; ++ix) {
-        const int64_t i = ix + b.dimx * iy;
-        if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
-        int64_t lo, hi;
-        sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, lo, hi);
-        if (hi <= lo) continue;
-        b.pp += hi - lo;
-        const int64_t t = (iy / b.tg.ty) * b.tg.ntx + ix / b.tg.tx;
-        tlo[(size_t)t] = std::min(tlo[(size_t)t], (int)lo);
-        thi[(size_t)t] = std::max(thi[(size_t)t], (int)hi);
-      }
-    for (int64_t t = 0; t < nt; ++t)
-      b.tile_len[(size_t)t] = thi[(size_t)t] > tlo[(size_t)t] ? thi[(size_t)t] - tlo[(size_t)t] : 0;
+  if (tile_len) {
+    b.tile_len = *tile_len;  // the caller keeps the request's pixel-pulse count (b.pp stays 0)
+  } else {
+    TileGrid tg;
+    request_tiles(ds, r, tg, b.tile_len, b.pp);
   }
   return SARBP_OK;
 }
 
 void reqbuf_free(ReqBuf &b) {
-  if (b.base) (void)hipFree(b.base);
+  if (b.base) {
+    if (b.pool)
+      b.pool->put(b.base, b.cap);
+    else
+      (void)hipFree(b.base);
+  }
   b.base = nullptr;
 }
 

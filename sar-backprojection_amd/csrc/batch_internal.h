@@ -1,14 +1,30 @@
 // Host-side structures shared by batched focusing (batch.hip) and the serving loop (server.hip).
 #pragma once
 
+#include <map>
 #include <vector>
 
 #include "sarbp_internal.h"
+
+// Device blocks kept for reuse by one owner thread (the serving loop allocates and releases one
+// block per request; hipMalloc/hipFree on every request would sit on its latency path).  Blocks
+// are sized in 1 MiB steps; a request takes the smallest cached block that is large enough and at
+// most twice its size.  At most `limit` bytes stay cached.
+struct DevicePool {
+  std::multimap<size_t, char *> blocks;
+  size_t cached = 0;
+  size_t limit = (size_t)2 << 30;
+  char *get(size_t bytes, size_t *cap);
+  void put(char *p, size_t cap);
+  void release();
+};
 
 // Device buffers of one request: one allocation holding pixels, outputs and (serving) state.
 struct ReqBuf {
   char *base = nullptr;
   int64_t bytes = 0;
+  size_t cap = 0;                 // size of the block at base
+  DevicePool *pool = nullptr;     // the block returns here (or is freed when null)
   int64_t n = 0, dimx = 0, dimy = 0;
   int precision = 0, accum = 0;
   TileGrid tg{};
@@ -18,12 +34,18 @@ struct ReqBuf {
   double2 *out = nullptr;
   int32_t *ncnt = nullptr;
   BpState *state = nullptr;
-  int64_t pp = 0;                 // pixel-pulse pairs of the request
+  int64_t pp = 0;                 // pixel-pulse pairs of the request (0 when tile_len was given)
   std::vector<int> tile_len;      // union length of each tile (pulses)
 };
 
-// Allocate and upload one request (asynchronously on st; the caller synchronises).
-int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b, hipStream_t st);
+// Tile grid, per-tile union lengths (pulses) and pixel-pulse pairs of a request (host, integers).
+void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
+                   int64_t &pp);
+// Allocate and upload one request (asynchronously on st; the caller synchronises).  pool: where
+// the device block comes from (null: hipMalloc); tile_len: precomputed by request_tiles (null:
+// computed here).
+int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b, hipStream_t st,
+                  DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 

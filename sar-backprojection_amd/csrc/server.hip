@@ -43,6 +43,7 @@ struct ServerReq {
   int max_len = 0;   // longest tile union (pulses)
   int64_t need = 0;  // device bytes while admitted
   std::vector<BpState> saved;  // partial state of a preempted request
+  std::vector<int> tile_len;   // per-tile union lengths, computed at submission (caller's thread)
   bool preempt = false;
   double submit_s = 0, admit_s = -1, done_s = -1;
   int32_t rounds = 0, preemptions = 0;
@@ -106,6 +107,7 @@ struct sarbp_server {
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ItemBuffers ib;
+  DevicePool pool;  // request blocks (scheduling thread only)
   int64_t bytes_in_use = 0;
   double rate = 3e11;  // pixel-pulse pairs per second, running estimate
   Clock::time_point t0 = Clock::now();
@@ -131,6 +133,7 @@ struct sarbp_server {
     std::vector<int32_t>().swap(r->index);
     std::vector<int32_t>().swap(r->sint);
     std::vector<BpState>().swap(r->saved);
+    std::vector<int>().swap(r->tile_len);
     r->req.x = r->req.y = r->req.z = nullptr;
     r->req.mask = nullptr;
     r->req.index = r->req.sint = nullptr;
@@ -138,22 +141,21 @@ struct sarbp_server {
 };
 
 bool sarbp_server::admit(ServerReq *r) {
-  int rc = reqbuf_create(r->ds, r->req, true, default_tiling(r->precision).accum, r->buf, st);
+  // the uploads are ordered before the round's launch on the same stream; only a saved state,
+  // whose host copy is dropped below, needs the stream drained first
+  int rc = reqbuf_create(r->ds, r->req, true, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len);
   if (rc) return false;
   if (!r->saved.empty()) {
     hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), r->saved.size() * sizeof(BpState),
                                   hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
+      (void)hipStreamSynchronize(st);
       reqbuf_free(r->buf);
       return false;
     }
   }
-  if (hipStreamSynchronize(st) != hipSuccess) {
-    reqbuf_free(r->buf);
-    return false;
-  }
-  r->saved.clear();
-  r->saved.shrink_to_fit();
+  std::vector<BpState>().swap(r->saved);
   r->on_device = true;
   r->max_len = 0;
   for (int v : r->buf.tile_len) r->max_len = std::max(r->max_len, v);
@@ -261,6 +263,11 @@ void sarbp_server::run() {
     stats.gpu_busy_s += ms * 1e-3;
     if (ms > 0.05f && round_pp > 0) rate = 0.7 * rate + 0.3 * (round_pp * 0.8 / (ms * 1e-3));  // lanes ~80 % active
     const bool failed = rc || e != hipSuccess;
+    if (failed) {  // nothing of this round may still run when its blocks return to the pool
+      lk.unlock();
+      (void)hipStreamSynchronize(st);
+      lk.lock();
+    }
     for (ServerReq *r : round) {
       r->rounds++;
       if (failed) {
@@ -390,11 +397,9 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   const int64_t nn = req->n;
   r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) + up(nn * (int64_t)sizeof(BpState));
   if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
-  for (size_t i = 0; i < n; ++i) {
-    if (r->mask[i] != 1 || r->sint[i] == -9999) continue;
-    int64_t lo, hi;
-    sarbp_pulse_range(r->index[i], r->sint[i], ds->naz, lo, hi);
-    r->pp += hi - lo;
+  {
+    TileGrid tg;
+    request_tiles(ds, r->req, tg, r->tile_len, r->pp);
   }
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return sarbp_fail(SARBP_ESTATE, "server is shutting down");
@@ -502,7 +507,9 @@ int sarbp_server_destroy(sarbp_server *s) {
     s->cv_work.notify_all();
   }
   if (s->started && s->th.joinable()) s->th.join();
+  if (s->st) (void)hipStreamSynchronize(s->st);
   s->ib.release();
+  s->pool.release();
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
   if (s->st) (void)hipStreamDestroy(s->st);
