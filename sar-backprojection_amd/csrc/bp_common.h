@@ -95,8 +95,10 @@ __device__ __forceinline__ void f64_acc(double r2, double sn, double cs, float2 
 #pragma clang fp contract(off)
   const double t1r = r2 * (double)d.x;       // :144, R2 * raw
   const double t1i = r2 * (double)d.y;
-  acc_re += t1r * cs + t1i * sn;             // t1r*cs - t1i*(-sn)
-  acc_im += t1i * cs - t1r * sn;             // t1r*(-sn) + t1i*cs
+  // the two products of each component go into the running sum with fused multiply-adds (one
+  // rounding each instead of three: as accurate or more than t1r*cs + t1i*sn then +=)
+  acc_re = fma(t1r, cs, fma(t1i, sn, acc_re));    // + t1r*cs - t1i*(-sn)
+  acc_im = fma(t1i, cs, fma(-t1r, sn, acc_im));   // + t1r*(-sn) + t1i*cs
 }
 
 // One pixel-pulse term against the pulse column col.  Returns false when the range bin falls
