@@ -24,28 +24,14 @@ for _p in (ROOT, os.path.join(ROOT, "sar-backprojection_amd")):
 
 import numpy as np  # noqa: E402
 
+from sarbp.metrics import split_error_report  # noqa: E402
+
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_PP = 8.0     # one complex64 RawData sample gathered per pixel-pulse pair
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def phase_amp_errors(test, ref):
-    """Wrapped phase error (compare_phase.py:50-56, 91) over |a| > 1e-6 and over |a| > 1e-5*max
-    (choerence_measure.py:20, 195-197), and relative amplitude errors."""
-    a_t, a_r = np.abs(test), np.abs(ref)
-    out = {}
-    for name, thr in (("abs1e-6", 1e-6), ("rel1e-5max", 1e-5 * float(a_r.max()))):
-        m = (a_t > thr) & (a_r > thr)
-        d = np.angle(np.exp(1j * (np.angle(test[m]) - np.angle(ref[m]))))
-        out[f"phase_rms_rad_{name}"] = float(np.sqrt(np.mean(d ** 2))) if m.any() else None
-        out[f"phase_max_rad_{name}"] = float(np.abs(d).max()) if m.any() else None
-    m = a_r > 1e-5 * float(a_r.max())
-    out["amp_rel_rmse"] = float(np.sqrt(np.sum((a_t - a_r) ** 2) / np.sum(a_r ** 2)))
-    out["amp_rel_max_rel1e-5max"] = float((np.abs(a_t[m] - a_r[m]) / a_r[m]).max()) if m.any() else None
-    return out
 
 
 def time_precision(scene, precision, steps, warmup):
@@ -125,38 +111,16 @@ def small_scene_leg(cores):
     nz = rcnt > 0
     out["f64_counts_exact"] = bool(np.array_equal(cnt, rcnt))
     out["f64_max_rel_err"] = float((np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])).max())
-    out["split_vs_oracle"] = phase_amp_errors(res["split"][0], ref)
+    out["split_vs_oracle"] = split_error_report(res["split"][0], ref)
     return out
 
 
-def target_peaks(sc, img_split, img_f64, half_x=3, half_y=2):
-    """Point-target responses: for each simulated target the peak of |image| in a
-    window around its pixel, split vs FP64: same peak pixel, relative peak-height difference."""
-    a_s = np.abs(img_split.reshape(sc.shape, order="F"))
-    a_d = np.abs(img_f64.reshape(sc.shape, order="F"))
-    nx, ny = sc.shape
-    dx = float(sc.x[1, 0] - sc.x[0, 0])
-    dy = float(sc.y[0, 1] - sc.y[0, 0])
-    med = float(np.median(a_d))
-    same, rel, used = 0, [], 0
-    for tx_, ty_ in sc.targets[:, :2]:
-        ix = int(round((tx_ - sc.x[0, 0]) / dx))
-        iy = int(round((ty_ - sc.y[0, 0]) / dy))
-        if ix - half_x < 0 or ix + half_x >= nx or iy - half_y < 0 or iy + half_y >= ny:
-            continue
-        ws = a_s[ix - half_x:ix + half_x + 1, iy - half_y:iy + half_y + 1]
-        wd = a_d[ix - half_x:ix + half_x + 1, iy - half_y:iy + half_y + 1]
-        if wd.max() <= 3 * med:  # target below the noise (or outside every aperture)
-            continue
-        used += 1
-        ps, pd = np.unravel_index(np.argmax(ws), ws.shape), np.unravel_index(np.argmax(wd), wd.shape)
-        same += int(ps == pd)
-        rel.append(abs(float(ws.max()) / float(wd.max()) - 1.0))
-    rel = np.asarray(rel)
-    return {"targets": int(sc.targets.shape[0]), "targets_above_noise": used,
-            "same_peak_pixel_fraction": same / max(used, 1),
-            "peak_height_rel_err_rms": float(np.sqrt(np.mean(rel ** 2))) if used else None,
-            "peak_height_rel_err_max": float(rel.max()) if used else None}
+def target_peaks(sc, img_split, img_f64):
+    """Point-target responses of the split image against the FP64 image (sarbp.metrics.target_peaks)."""
+    from sarbp import metrics
+    return metrics.target_peaks(img_split.reshape(sc.shape, order="F"), img_f64.reshape(sc.shape, order="F"),
+                                sc.targets[:, :2], float(sc.x[0, 0]), float(sc.y[0, 0]),
+                                float(sc.x[1, 0] - sc.x[0, 0]), float(sc.y[0, 1] - sc.y[0, 0]))
 
 
 def main():
@@ -240,7 +204,7 @@ def main():
         extra[f"{p}_fp64_valu_frac"] = 26.0 * pp / r["kernel_s"] / 78.6e12
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
-        extra["split_vs_f64_errors"] = phase_amp_errors(outs["split"], outs["f64"])
+        extra["split_vs_f64_errors"] = split_error_report(outs["split"], outs["f64"])
         if sc.targets is not None:
             extra["split_vs_f64_target_peaks"] = target_peaks(sc, outs["split"], outs["f64"])
 

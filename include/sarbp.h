@@ -173,8 +173,9 @@ int sarbp_eval_pairs(sarbp_dataset *ds, const double *x, const double *y, const 
  * sums of every pixel stay on the device between rounds (the per-request checkpoint), so a request
  * can be preempted (its state copied to the host and its device memory released) and resumed later
  * with a bit-identical result; results never depend on which requests share a round.
- * A split-precision request whose a-priori phase-error estimate exceeds phase_err_limit_rad is
- * focused in FP64 instead.  No counterpart in the reference, whose drivers process one dataset
+ * A split-precision request whose a-priori estimate of its RMS phase error (against the FP64
+ * result, over pixels with |a| > 1e-5 max|a|) exceeds phase_err_limit_rad is focused in FP64
+ * instead; the estimate comes from the pixels' tile geometry and bounds the measured error.  No counterpart in the reference, whose drivers process one dataset
  * pass at a time (main_gpu.py:415-575).
  */
 typedef struct sarbp_server sarbp_server;
@@ -196,7 +197,7 @@ typedef struct sarbp_server_config {
 typedef struct sarbp_request_info {
   int32_t state;             /* SARBP_REQ_* */
   int32_t precision;         /* precision actually used (after the FP64 fallback decision) */
-  double phase_err_est_rad;  /* a-priori split phase-error estimate */
+  double phase_err_est_rad;  /* a-priori estimate of the split result's RMS phase error */
   double submit_s, admit_s, done_s; /* server clock (seconds since server creation) */
   int64_t pp;                /* pixel-pulse pairs of the request */
   int32_t rounds;            /* scheduling rounds the request took part in */

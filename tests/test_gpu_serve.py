@@ -8,7 +8,7 @@ import time
 import numpy as np
 import pytest
 
-from sarbp import _lib, synth
+from sarbp import _lib, metrics, synth
 from sarbp.batch import Batch, Request, patch_requests
 from sarbp.core import Dataset
 from sarbp.serve import Server
@@ -115,4 +115,30 @@ def test_split_falls_back_to_fp64_when_estimate_exceeds_limit(gpu, world):
     print("estimates", i1["phase_err_est_rad"], i2["phase_err_est_rad"], flush=True)
     assert i1["precision"] == _lib.PREC_F64 and i1["phase_err_est_rad"] > 1e-3
     assert i2["precision"] == _lib.PREC_SPLIT and i2["phase_err_est_rad"] < 1e-3
+    srv.close()
+
+
+def test_split_estimate_bounds_measured_phase_error(gpu, world):
+    """The a-priori estimate behind the FP64 fallback against the measured RMS phase error of the
+    split result (sarbp.metrics.split_error_report against the FP64 result of the same pixels), on
+    grids 1 m to 6 m apart along x: the tile rows' span, hence the range-offset term, grows with the
+    spacing."""
+    sc, ds, _ = world
+    srv = Server(phase_err_limit_rad=1e9)  # no fallback: every split request runs split
+    for step in (1, 2, 4, 6):
+        sub = sc.sub(0, 512, 100, 164)
+        pick = (slice(0, 512, step), slice(0, 64))
+        f = lambda a: np.asfortranarray(a[pick]).flatten(order="F")  # noqa: E731
+        nx = len(range(0, 512, step))
+        args = [f(a) for a in (sub.x, sub.y, sub.z, sub.mask, sub.index, sub.sint)]
+        print("case estimate grid step", step, "m, dimx", nx, flush=True)
+        h1 = srv.submit(ds, Request(*args, dimx=nx, precision="split"))
+        h2 = srv.submit(ds, Request(*args, dimx=nx, precision="f64"))
+        assert h1.wait(60) and h2.wait(60)
+        est = h1.info()["phase_err_est_rad"]
+        assert h1.info()["precision"] == _lib.PREC_SPLIT
+        e = metrics.split_error_report(h1.result(), h2.result())
+        print("step", step, "estimate", est, "measured", e, flush=True)
+        assert e["phase_rms_rad_rel1e-5max"] <= est
+        assert e["phase_rms_rad_rel1e-5max"] <= 1e-3
     srv.close()
