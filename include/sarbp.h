@@ -173,6 +173,11 @@ int sarbp_eval_pairs(sarbp_dataset *ds, const double *x, const double *y, const 
  * sums of every pixel stay on the device between rounds (the per-request checkpoint), so a request
  * can be preempted (its state copied to the host and its device memory released) and resumed later
  * with a bit-identical result; results never depend on which requests share a round.
+ * A request too small to fill the device on its own (few tiles) has each tile's pulses split into
+ * up to max_segments segments, focused in parallel by separate wavefronts in its first round and
+ * added in segment order by a second kernel: its latency drops by up to that factor, and its
+ * result still depends only on the request (not on the mix), though it can differ in the last bits
+ * from the same request focused with its pulses in one sequence.
  * A split-precision request whose a-priori estimate of its RMS phase error (against the FP64
  * result, over pixels with |a| > 1e-5 max|a|) exceeds phase_err_limit_rad is focused in FP64
  * instead; the estimate comes from the pixels' tile geometry and bounds the measured error.  No counterpart in the reference, whose drivers process one dataset
@@ -184,8 +189,9 @@ typedef struct sarbp_server_config {
   int64_t hbm_budget_bytes;   /* admission budget for request buffers (0: 200 GB) */
   double quantum_ms;          /* target GPU time of one scheduling round (0: 20 ms) */
   int32_t max_active;         /* admitted requests at once (0: 256) */
-  int32_t pulse_chunk;        /* pulse granularity of round spans and LDS refills (0: 64) */
+  int32_t pulse_chunk;        /* pulse granularity of round spans and LDS refills (0: 32) */
   double phase_err_limit_rad; /* split -> FP64 fallback threshold (0: 1e-3) */
+  int32_t max_segments;       /* pulse segments per tile of a small request (0: 16; 1: never split) */
 } sarbp_server_config;
 
 #define SARBP_REQ_QUEUED 0
@@ -202,6 +208,7 @@ typedef struct sarbp_request_info {
   int64_t pp;                /* pixel-pulse pairs of the request */
   int32_t rounds;            /* scheduling rounds the request took part in */
   int32_t preemptions;
+  int32_t segments;          /* pulse segments per tile (1: the tiles' pulses run in order) */
 } sarbp_request_info;
 
 typedef struct sarbp_server_stats {

@@ -87,7 +87,7 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
     tile_len[(size_t)t] = thi[(size_t)t] > tlo[(size_t)t] ? thi[(size_t)t] - tlo[(size_t)t] : 0;
 }
 
-int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b,
+int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
                   hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
@@ -108,7 +108,7 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
   const size_t n = (size_t)r.n;
   size_t off[10];
   size_t o = 0;
-  const size_t sz[9] = {n * 8, n * 8, n * 8, n, n * 4, n * 4, n * 16, n * 4, with_state ? n * sizeof(BpState) : 0};
+  const size_t sz[9] = {n * 8, n * 8, n * 8, n, n * 4, n * 4, n * 16, n * 4, n * sizeof(BpState) * (size_t)state_slots};
   for (int k = 0; k < 9; ++k) {
     off[k] = o;
     o += up(sz[k]);
@@ -138,7 +138,7 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, in
   b.sint = (int32_t *)(p + off[5]);
   b.out = (double2 *)(p + off[6]);
   b.ncnt = (int32_t *)(p + off[7]);
-  b.state = with_state ? (BpState *)(p + off[8]) : nullptr;
+  b.state = state_slots > 0 ? (BpState *)(p + off[8]) : nullptr;
   const void *src[6] = {r.x, r.y, r.z, r.mask, r.index, r.sint};
   void *dst[6] = {b.x, b.y, b.z, b.mask, b.index, b.sint};
   for (int k = 0; k < 6 && n; ++k) {
@@ -269,7 +269,7 @@ int batch_build(sarbp_batch *b, int waves) {
     for (int64_t t = 0; t < nt; t += waves) {
       int64_t c = 0;
       for (int64_t u = t; u < std::min(nt, t + waves); ++u) c = std::max<int64_t>(c, q.tile_len[(size_t)u]);
-      work.push_back(WorkItemCost{BatchItem{(int32_t)r, (int32_t)t, 0, INT32_MAX}, c});
+      work.push_back(WorkItemCost{BatchItem{(int32_t)r, (int32_t)t, 0, INT32_MAX, -1}, c});
     }
   }
   std::vector<BatchItem> items;
@@ -323,7 +323,7 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
   b->ds = ds;
   b->reqs.resize((size_t)nreq);
   for (int32_t r = 0; r < nreq; ++r) {
-    int rc = reqbuf_create(ds, reqs[r], false, default_tiling(reqs[r].precision).accum, b->reqs[(size_t)r],
+    int rc = reqbuf_create(ds, reqs[r], 0, default_tiling(reqs[r].precision).accum, b->reqs[(size_t)r],
                            ds->stream);
     if (rc) {
       sarbp_batch_destroy(b);

@@ -41,10 +41,11 @@ struct ReqBuf {
 // Tile grid, per-tile union lengths (pulses) and pixel-pulse pairs of a request (host, integers).
 void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
                    int64_t &pp);
-// Allocate and upload one request (asynchronously on st; the caller synchronises).  pool: where
-// the device block comes from (null: hipMalloc); tile_len: precomputed by request_tiles (null:
-// computed here).
-int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, bool with_state, int accum, ReqBuf &b, hipStream_t st,
+// Allocate and upload one request (asynchronously on st; the caller synchronises).  state_slots:
+// BpState entries per pixel (0: none; serving: 1 for the state between rounds, or one per pulse
+// segment).  pool: where the device block comes from (null: hipMalloc); tile_len: precomputed by
+// request_tiles (null: computed here).
+int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b, hipStream_t st,
                   DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);

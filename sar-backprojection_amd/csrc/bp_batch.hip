@@ -21,15 +21,16 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
                                            int64_t tile, int rec0, int C) {
   const BpGeom g = r.g;
   const TileLane t = tile_lane(g, r.px, r.tg, tile, threadIdx.x & 63);
-  const bool first = it.p_off == 0;
+  const bool segment = it.slot >= 0;  // partial sums of one pulse segment, into slot it.slot
+  const bool first = segment || it.p_off == 0;
   if (t.wlo >= t.whi) {  // nothing to focus in this tile
-    if (first && t.valid) store_mean(r.out, r.ncnt, t.i, 0.0, 0.0, 0);
+    if (!segment && first && t.valid) store_mean(r.out, r.ncnt, t.i, 0.0, 0.0, 0);
     return;
   }
   const int p0 = t.wlo + it.p_off;
-  if (p0 >= t.whi) return;  // this tile finished in an earlier range
+  if (p0 >= t.whi) return;  // this tile finished in an earlier range (a segment: its slot stays zero)
   const int p1 = (it.p_end == INT32_MAX || it.p_end >= t.whi - t.wlo) ? t.whi : t.wlo + it.p_end;
-  const bool last = p1 >= t.whi;
+  const bool last = segment || p1 >= t.whi;  // final sums of this item's pulses
   double ar = 0.0, ai = 0.0;
   float cr = 0.f, ci = 0.f;
   int32_t cnt = 0;
@@ -63,7 +64,16 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
     split_span<1>(g, t, s, p0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);
   }
   if (!t.valid) return;
-  if (last) {
+  if (segment) {
+    BpState st;
+    st.ar = ar;
+    st.ai = ai;
+    st.cr = 0.f;
+    st.ci = 0.f;
+    st.cnt = cnt;
+    st.pad = 0;
+    r.state[(int64_t)it.slot * r.px.n + t.i] = st;
+  } else if (last) {
     store_mean(r.out, r.ncnt, t.i, ar, ai, cnt);
   } else {
     BpState st;
@@ -78,7 +88,10 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
 }
 
 // C pulses of records per refill, up to max_rows rows per tile: C * max_rows records per wavefront.
-__global__ __launch_bounds__(256) void k_bp_batch(const BatchReqDev *__restrict__ reqs,
+// Compiled for four wavefronts per SIMD (at most 128 VGPRs): the FP64 and split paths together
+// need 141 registers unbounded (three waves); five or six waves spill in the loops and are slower
+// (64 mixed requests: 29.0 ms at four waves, 32.6 ms at three, 41 ms at five).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bp_batch(const BatchReqDev *__restrict__ reqs,
                                                   const BatchItem *__restrict__ items,
                                                   const int32_t *__restrict__ seg, int32_t *heads, int C,
                                                   int max_rows) {
@@ -107,7 +120,32 @@ __global__ __launch_bounds__(256) void k_bp_batch(const BatchReqDev *__restrict_
   }
 }
 
+__global__ __launch_bounds__(256) void k_bp_combine(const BpState *__restrict__ part, int nseg, int64_t n,
+                                                    double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ar = 0.0, ai = 0.0;
+  int32_t cnt = 0;
+  for (int k = 0; k < nseg; ++k) {  // fixed order: the result does not depend on segment timing
+    const BpState s = part[(int64_t)k * n + i];
+    ar += s.ar;
+    ai += s.ai;
+    cnt += s.cnt;
+  }
+  store_mean(out, ncnt, i, ar, ai, cnt);
+}
+
 }  // namespace
+
+int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out, int32_t *ncnt, hipStream_t st) {
+  if (n <= 0) return SARBP_OK;
+  if (nseg < 1) return sarbp_fail(SARBP_EINVAL, "segment count must be positive");
+  if ((n + 255) / 256 > INT32_MAX) return sarbp_fail(SARBP_EINVAL, "too many pixels for one combine launch");
+  hipLaunchKernelGGL(k_bp_combine, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, part, nseg, n, out, ncnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return sarbp_hip_fail(e, "combine kernel launch");
+  return SARBP_OK;
+}
 
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
   return (size_t)waves * (size_t)chunk * (size_t)max_rows * sizeof(PulseRec);

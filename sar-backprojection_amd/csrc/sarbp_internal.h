@@ -186,7 +186,8 @@ struct BatchReqDev {
   PixelsView px;
   double2 *out;
   int32_t *ncnt;
-  BpState *state;    // partial sums between launches (serving); may be null for one-shot batches
+  BpState *state;    // partial sums between launches (serving), or the pulse segments' partial
+                     // sums (slot k of pixel i at state[k * n + i]); may be null for one-shot batches
   TileGrid tg;
   int32_t precision;  // SARBP_PREC_*
   int32_t accum;      // split path accumulation (0 Kahan, 1 FP64 fold)
@@ -194,12 +195,16 @@ struct BatchReqDev {
 
 // Work item: tiles [tile_lo, tile_lo + W) of one request (one per wavefront of the block), pulses
 // [wlo + p_off, wlo + p_end) of each tile's union (p_end = INT32_MAX: to the end).  p_off and a
-// finite p_end are multiples of the pulse chunk C.
+// finite p_end are multiples of the pulse chunk C.  slot < 0: the item continues the pixels' sums
+// from the saved state (p_off > 0) or from zero, and writes the means when it reaches the end of
+// the union.  slot >= 0: a pulse segment; its sums start from zero and go to partial slot `slot`
+// (combined afterwards by sarbp_launch_combine).
 struct BatchItem {
   int32_t req;
   int32_t tile_lo;
   int32_t p_off;
   int32_t p_end;
+  int32_t slot;
 };
 
 constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
@@ -211,3 +216,6 @@ constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
 int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
                        int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks, hipStream_t st);
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);
+// Pixel means from the partial sums of nseg pulse segments (part[k * n + i], added in the order
+// k = 0 .. nseg-1): out[i], ncnt[i] for i < n.
+int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out, int32_t *ncnt, hipStream_t st);

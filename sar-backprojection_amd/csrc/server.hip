@@ -4,9 +4,11 @@
 // device memory within the budget are free, then focuses every admitted request in ONE launch of
 // the persistent batch kernel (bp_batch.hip), each tile advanced by the round's pulse span.  Partial
 // per-pixel sums stay in device memory between rounds; completed requests are copied out and their
-// memory released, preempted ones have their partial state saved on the host.  Results do not
-// depend on the mix or on the round boundaries: the per-pixel arithmetic and its order are fixed by
-// the request alone.
+// memory released, preempted ones have their partial state saved on the host.  A request with too
+// few tiles to fill the device is instead focused whole in its first round, each tile's pulses
+// split into segments run by separate wavefronts, whose partial sums a second kernel adds in
+// segment order.  Results do not depend on the mix or on the round boundaries: the per-pixel
+// arithmetic and its order are fixed by the request alone.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -48,7 +50,24 @@ struct ServerReq {
   double submit_s = 0, admit_s = -1, done_s = -1;
   int32_t rounds = 0, preemptions = 0;
   int64_t pp = 0;
+  int nseg = 1;     // pulse segments per tile (> 1: focused whole in its first round)
+  int seg_len = 0;  // pulses per segment (a multiple of the chunk)
 };
+
+// Pulse segments per tile for a request whose live tiles (union length > 0) leave most of the
+// device idle: about `target` wavefronts in all, at most max_seg segments, each at least four
+// pulse chunks long.  Depends only on the request, the device and the configuration.
+int segments_for(int64_t live_tiles, int max_len, int64_t target, int chunk, int max_seg, int *seg_len) {
+  *seg_len = 0;
+  if (live_tiles <= 0 || max_seg <= 1) return 1;
+  int64_t k = std::min<int64_t>(max_seg, target / live_tiles);
+  k = std::min<int64_t>(k, max_len / (4 * chunk));
+  if (k <= 1) return 1;
+  const int64_t per = (max_len + k - 1) / k;
+  const int len = (int)((per + chunk - 1) / chunk * chunk);
+  *seg_len = len;
+  return (max_len + len - 1) / len;
+}
 
 // A-priori RMS phase error (rad) of a request's split result against its FP64 result, over the
 // pixels with |a| > 1e-5 max|a|.  Per-term model: the FP32 error of the range offset
@@ -107,6 +126,7 @@ struct sarbp_server {
   bool stop = false;
   bool started = false;
   int device = 0;
+  int64_t target_waves = 8192;  // wavefronts that fill the device (32 per CU)
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ItemBuffers ib;
@@ -146,8 +166,17 @@ struct sarbp_server {
 bool sarbp_server::admit(ServerReq *r) {
   // the uploads are ordered before the round's launch on the same stream; only a saved state,
   // whose host copy is dropped below, needs the stream drained first
-  int rc = reqbuf_create(r->ds, r->req, true, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len);
+  int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len);
   if (rc) return false;
+  if (r->nseg > 1 && r->buf.n) {  // segment slots of tiles shorter than the last segment stay zero
+    hipError_t e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(st);
+      reqbuf_free(r->buf);
+      return false;
+    }
+  }
+  if (r->off == 0) std::vector<BpState>().swap(r->saved);  // no progress to restore
   if (!r->saved.empty()) {
     hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), r->saved.size() * sizeof(BpState),
                                   hipMemcpyHostToDevice, st);
@@ -223,9 +252,10 @@ void sarbp_server::run() {
     }
     // round: every active request advances every unfinished tile by `span` pulses
     int64_t live_px = 0;
-    for (ServerReq *r : active) live_px += std::max<int64_t>(1, r->buf.n);
+    for (ServerReq *r : active)
+      if (r->nseg == 1) live_px += std::max<int64_t>(1, r->buf.n);
     const double budget_pp = rate * cfg.quantum_ms * 1e-3;
-    int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)live_px));
+    int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)std::max<int64_t>(1, live_px)));
     span = std::max(chunk, (span + chunk - 1) / chunk * chunk);
     std::vector<ServerReq *> round(active);
     std::vector<BatchReqDev> devs;
@@ -237,12 +267,24 @@ void sarbp_server::run() {
       devs.push_back(reqbuf_dev(r->ds, r->buf));
       if (r->precision == SARBP_PREC_SPLIT) max_rows = std::max(max_rows, r->buf.tg.ty);
       const int64_t nt = r->buf.tg.ntiles();
+      if (r->nseg > 1) {  // every segment of every tile, this round
+        for (int64_t t = 0; t < nt; ++t) {
+          const int len = r->buf.tile_len[(size_t)t];
+          for (int j = 0; j < r->nseg && j * r->seg_len < len; ++j) {
+            const int a = j * r->seg_len;
+            const int c = std::min(len, a + r->seg_len) - a;
+            work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, a, a + r->seg_len, j}, c});
+            round_pp += (int64_t)c * 64;
+          }
+        }
+        continue;
+      }
       const int p_end = r->off + span;
       for (int64_t t = 0; t < nt; ++t) {
         const int len = r->buf.tile_len[(size_t)t];
         if (len <= r->off) continue;  // finished (or empty: its zeros are written at admission)
         const int c = std::min(len, p_end) - r->off;
-        work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, r->off, p_end}, c});
+        work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, r->off, p_end, -1}, c});
         round_pp += (int64_t)c * 64;
       }
     }
@@ -257,6 +299,9 @@ void sarbp_server::run() {
       e = hipEventRecord(e0, st);
       if (e == hipSuccess)
         rc = sarbp_launch_batch(device, ib.d_reqs, ib.d_items, ib.d_seg, ib.d_heads, 1, chunk, max_rows, 0, st);
+      for (ServerReq *r : round)  // segmented requests: means from the segments' partial sums
+        if (!rc && e == hipSuccess && r->nseg > 1)
+          rc = sarbp_launch_combine(r->buf.state, r->nseg, r->buf.n, r->buf.out, r->buf.ncnt, st);
       if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
       if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
       if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
@@ -279,7 +324,7 @@ void sarbp_server::run() {
         stats.failed++;
         continue;
       }
-      r->off += span;
+      r->off = r->nseg > 1 ? r->max_len : r->off + span;
       if (r->off >= r->max_len) {  // every tile wrote its means: copy the result out
         lk.unlock();
         hipError_t ce = hipSuccess;
@@ -329,14 +374,18 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
   if (c.hbm_budget_bytes <= 0) c.hbm_budget_bytes = (int64_t)200e9;
   if (c.quantum_ms <= 0) c.quantum_ms = 20.0;
   if (c.max_active <= 0) c.max_active = 256;
-  if (c.pulse_chunk <= 0) c.pulse_chunk = 64;
+  if (c.pulse_chunk <= 0) c.pulse_chunk = 32;
   if (c.phase_err_limit_rad <= 0) c.phase_err_limit_rad = 1e-3;
+  if (c.max_segments <= 0) c.max_segments = 16;
   if (c.pulse_chunk % 8 || c.pulse_chunk > 1024) {
     delete s;
     return sarbp_fail(SARBP_EINVAL, "pulse_chunk must be a multiple of 8, at most 1024");
   }
   s->cfg = c;
   hipError_t e = hipGetDevice(&s->device);
+  int ncu = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device);
+  if (e == hipSuccess && ncu > 0) s->target_waves = (int64_t)ncu * 32;
   if (e == hipSuccess) e = hipStreamCreate(&s->st);
   if (e == hipSuccess) e = hipEventCreate(&s->e0);
   if (e == hipSuccess) e = hipEventCreate(&s->e1);
@@ -395,15 +444,26 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   }
   r->req.precision = r->precision;
   r->req.tile_x = r->precision == req->precision ? tx : 0;
-  // device bytes while admitted: pixels + outputs + state (each array 256-byte aligned)
-  auto up = [](int64_t v) { return (v + 255) / 256 * 256; };
-  const int64_t nn = req->n;
-  r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) + up(nn * (int64_t)sizeof(BpState));
-  if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
   {
     TileGrid tg;
     request_tiles(ds, r->req, tg, r->tile_len, r->pp);
   }
+  {
+    int max_len = 0;
+    int64_t live = 0;
+    for (int v : r->tile_len) {
+      max_len = std::max(max_len, v);
+      live += v > 0;
+    }
+    r->nseg = segments_for(live, max_len, s->target_waves, s->cfg.pulse_chunk, s->cfg.max_segments, &r->seg_len);
+  }
+  // device bytes while admitted: pixels + outputs + state or segment slots (each array 256-byte
+  // aligned)
+  auto up = [](int64_t v) { return (v + 255) / 256 * 256; };
+  const int64_t nn = req->n;
+  r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) +
+            up(nn * (int64_t)sizeof(BpState) * r->nseg);
+  if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return sarbp_fail(SARBP_ESTATE, "server is shutting down");
   r->id = s->next_id++;
@@ -489,6 +549,7 @@ int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info) {
   info->pp = r->pp;
   info->rounds = r->rounds;
   info->preemptions = r->preemptions;
+  info->segments = r->nseg;
   return SARBP_OK;
 }
 

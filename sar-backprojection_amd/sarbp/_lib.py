@@ -47,13 +47,14 @@ _RP = ctypes.POINTER(Request)
 class ServerConfig(ctypes.Structure):
     """sarbp_server_config (include/sarbp.h); zero fields take the library defaults."""
     _fields_ = [("hbm_budget_bytes", _i64), ("quantum_ms", _d), ("max_active", _i32), ("pulse_chunk", _i32),
-                ("phase_err_limit_rad", _d)]
+                ("phase_err_limit_rad", _d), ("max_segments", _i32)]
 
 
 class RequestInfo(ctypes.Structure):
     """sarbp_request_info (include/sarbp.h)."""
     _fields_ = [("state", _i32), ("precision", _i32), ("phase_err_est_rad", _d), ("submit_s", _d),
-                ("admit_s", _d), ("done_s", _d), ("pp", _i64), ("rounds", _i32), ("preemptions", _i32)]
+                ("admit_s", _d), ("done_s", _d), ("pp", _i64), ("rounds", _i32), ("preemptions", _i32),
+                ("segments", _i32)]
 
 
 class ServerStats(ctypes.Structure):

@@ -61,11 +61,16 @@ class Handle:
 
 
 class Server:
+    """sarbp_server_*: zero arguments take the library defaults (200 GB budget, 20 ms rounds, 256
+    active requests, 32-pulse chunks, 1e-3 rad split fallback limit, up to 16 pulse segments per
+    tile for requests too small to fill the device; max_segments=1 runs every tile's pulses in one
+    sequence, as the one-shot batch kernel does)."""
     _close_order = 0  # servers, then batches/scenes, then datasets
-    def __init__(self, hbm_budget_bytes=0, quantum_ms=0.0, max_active=0, pulse_chunk=0, phase_err_limit_rad=0.0):
+    def __init__(self, hbm_budget_bytes=0, quantum_ms=0.0, max_active=0, pulse_chunk=0, phase_err_limit_rad=0.0,
+                 max_segments=0):
         _lib.require_device()
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), float(quantum_ms), int(max_active), int(pulse_chunk),
-                                float(phase_err_limit_rad))
+                                float(phase_err_limit_rad), int(max_segments))
         h = ctypes.c_void_p()
         check(_lib.load().sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)), "sarbp_server_create")
         self._h = h
@@ -183,6 +188,8 @@ def main(argv=None):
     ap.add_argument("--datasets", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--quantum-ms", type=float, default=20.0)
+    ap.add_argument("--max-segments", type=int, default=0, help="pulse segments per tile of small requests "
+                    "(0: library default; 1: off)")
     ap.add_argument("--small", action="store_true")
     a = ap.parse_args(argv)
     from . import synth
@@ -197,7 +204,7 @@ def main(argv=None):
         scenes.append(base)
         log(f"dataset {k} resident")
     pool = make_pool(scenes)
-    srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms)
+    srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms, max_segments=a.max_segments)
     results = []
     for rate in [float(r) for r in a.rates.split(",") if r]:
         res = poisson_run(srv, datasets, pool, rate, a.duration, log=log)

@@ -61,3 +61,37 @@ def test_null_arguments_rejected_without_device_work():
     assert b"null" in L.sarbp_last_error()
     assert L.sarbp_scene_create(None, None, None, None, None, None, None, 0, 0, ctypes.byref(h)) == _lib.EINVAL
     assert L.sarbp_dataset_destroy(None) == 0
+
+
+STRUCTS = [("sarbp_request", "Request"), ("sarbp_server_config", "ServerConfig"),
+           ("sarbp_request_info", "RequestInfo"), ("sarbp_server_stats", "ServerStats")]
+
+
+def test_ctypes_struct_layouts_match_header(tmp_path):
+    """Every structure the bindings pass across the ABI has the header's size and field offsets
+    (a C program built against include/sarbp.h prints them)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "sarbp.h"', "int main(void) {"]
+    for cname, pyname in STRUCTS:
+        lines.append(f'  printf("{pyname} size %zu\\n", sizeof({cname}));')
+        for f, _ in getattr(_lib, pyname)._fields_:
+            lines.append(f'  printf("{pyname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = {}
+    for ln in out:
+        if ln:
+            py, field, v = ln.split()
+            want[(py, field)] = int(v)
+    for _, pyname in STRUCTS:
+        cls = getattr(_lib, pyname)
+        assert ctypes.sizeof(cls) == want[(pyname, "size")], pyname
+        for f, _ in cls._fields_:
+            assert getattr(cls, f).offset == want[(pyname, f)], (pyname, f)

@@ -1,13 +1,16 @@
 """Serving loop: continuous batching, preemption/resume, admission budget, FP64 fallback (GPU).
 
 The bar for continuous batching: every request's result is bit-identical whether it ran alone or
-in a mix with others, and equal to the same request focused by the batch kernel alone.
+in a mix with others; with each tile's pulses in one sequence it also equals the same request
+focused by the batch kernel alone, and with the pulses of a small request split into segments its
+FP64 result stays within the oracle's parity bound.
 """
 import time
 
 import numpy as np
 import pytest
 
+from oracle import oracle_lib as O
 from sarbp import _lib, metrics, synth
 from sarbp.batch import Batch, Request, patch_requests
 from sarbp.core import Dataset
@@ -26,12 +29,7 @@ def world():
     ds2.close()
 
 
-def test_alone_equals_mix_equals_batch(gpu, world):
-    sc, ds, ds2 = world
-    reqs = patch_requests(sc, 8, seed=3, w_range=(16, 80), h_range=(8, 40), sint_range=(500, 4000))
-    dsets = [ds if k % 3 else ds2 for k in range(len(reqs))]
-    print("case serve mix of", len(reqs), flush=True)
-    srv = Server(quantum_ms=2.0)
+def _serve_alone_and_mixed(srv, dsets, reqs):
     alone = []
     for d, r in zip(dsets, reqs):
         h = srv.submit(d, r)
@@ -40,7 +38,20 @@ def test_alone_equals_mix_equals_batch(gpu, world):
     hs = [srv.submit(d, r) for d, r in zip(dsets, reqs)]
     for h in hs:
         assert h.wait(timeout=60)
+    return alone, hs
+
+
+def test_alone_equals_mix_equals_batch(gpu, world):
+    """Pulses in one sequence per tile (max_segments=1): the same bits alone, in a mix and in the
+    one-shot batch kernel."""
+    sc, ds, ds2 = world
+    reqs = patch_requests(sc, 8, seed=3, w_range=(16, 80), h_range=(8, 40), sint_range=(500, 4000))
+    dsets = [ds if k % 3 else ds2 for k in range(len(reqs))]
+    print("case serve mix of", len(reqs), flush=True)
+    srv = Server(quantum_ms=2.0, max_segments=1)
+    alone, hs = _serve_alone_and_mixed(srv, dsets, reqs)
     rounds = [h.info()["rounds"] for h in hs]
+    assert all(h.info()["segments"] == 1 for h in hs)
     for (a, ac), h in zip(alone, hs):
         b, bc = h.result(with_counts=True)
         assert np.array_equal(ac, bc) and np.array_equal(a, b)
@@ -58,11 +69,53 @@ def test_alone_equals_mix_equals_batch(gpu, world):
     srv.close()
 
 
+def test_segmented_requests(gpu, world):
+    """Small requests with their tiles' pulses split into segments (the default): the same bits
+    alone and in a mix; contributor counts equal to the sequential run's; FP64 results within the
+    parity bound of the oracle, split results within the split tolerances of the sequential ones."""
+    sc, ds, ds2 = world
+    reqs = patch_requests(sc, 8, seed=4, w_range=(16, 80), h_range=(8, 40), sint_range=(500, 4000))
+    dsets = [ds if k % 3 else ds2 for k in range(len(reqs))]
+    print("case serve segmented mix of", len(reqs), flush=True)
+    srv = Server(quantum_ms=2.0)
+    alone, hs = _serve_alone_and_mixed(srv, dsets, reqs)
+    segs = [h.info()["segments"] for h in hs]
+    rounds = [h.info()["rounds"] for h in hs]
+    print("segments", segs, "rounds", rounds, flush=True)
+    assert max(segs) > 1 and all(r == 1 for r, s in zip(rounds, segs) if s > 1)
+    for (a, ac), h in zip(alone, hs):
+        b, bc = h.result(with_counts=True)
+        assert np.array_equal(ac, bc) and np.array_equal(a, b)
+    srv.close()
+    seq = Server(quantum_ms=2.0, max_segments=1)
+    for k, (d, r) in enumerate(zip(dsets, reqs)):
+        h = seq.submit(d, r)
+        assert h.wait(timeout=60)
+        ref, rc = h.result(with_counts=True)
+        got, gc = alone[k]
+        assert np.array_equal(gc, rc)
+        if r.precision == "f64":
+            raw = sc.raw if d is ds else np.asfortranarray(sc.raw * np.complex64(1j))
+            o, oc = O.focus_flat(r.x, r.y, r.z, r.mask, r.index, r.sint, sc.tx, sc.ty, sc.tz, raw, sc.near, sc.dra,
+                                 sc.lam)
+            assert np.array_equal(gc, oc)
+            m = oc > 0
+            rel = np.abs(got[m] - o[m]) / np.abs(o[m])
+            print("request", k, "segments", segs[k], "f64 max rel err vs oracle", float(rel.max()),
+                  "vs sequential", float((np.abs(got[m] - ref[m]) / np.abs(ref[m])).max()), flush=True)
+            assert rel.max() <= 1e-10
+        else:
+            e = metrics.split_error_report(got, ref)
+            print("request", k, "segments", segs[k], "split vs sequential", e, flush=True)
+            assert e["phase_rms_rad_rel1e-5max"] <= 1e-4 and e["amp_rel_rmse"] <= 1e-5
+    seq.close()
+
+
 def test_preempt_and_resume_is_bit_identical(gpu, world):
     sc, ds, _ = world
     (req,) = patch_requests(sc, 1, seed=5, w_range=(200, 200), h_range=(64, 64), sint_range=(3800, 3800),
                             precisions=("split",))
-    srv = Server(quantum_ms=0.001, pulse_chunk=64)  # one 64-pulse chunk per round: many rounds
+    srv = Server(quantum_ms=0.001, pulse_chunk=64, max_segments=1)  # one 64-pulse chunk per round: many rounds
     ref = srv.submit(ds, req)
     assert ref.wait(timeout=120)
     h = srv.submit(ds, req)
@@ -85,7 +138,7 @@ def test_budget_admission_and_rejection(gpu, world):
     sc, ds, _ = world
     reqs = patch_requests(sc, 4, seed=8, w_range=(64, 64), h_range=(32, 32), sint_range=(1000, 1000))
     per_req = 64 * 32 * (8 * 3 + 1 + 4 * 2 + 16 + 4 + 32) + 8 * 256
-    srv = Server(hbm_budget_bytes=int(per_req * 1.5), quantum_ms=1.0)  # room for one request at a time
+    srv = Server(hbm_budget_bytes=int(per_req * 1.5), quantum_ms=1.0, max_segments=1)  # one request at a time
     hs = [srv.submit(ds, r) for r in reqs]
     for h in hs:
         assert h.wait(timeout=120)
