@@ -204,11 +204,13 @@ typedef struct sarbp_request_info {
   int32_t state;             /* SARBP_REQ_* */
   int32_t precision;         /* precision actually used (after the FP64 fallback decision) */
   double phase_err_est_rad;  /* a-priori estimate of the split result's RMS phase error */
-  double submit_s, admit_s, done_s; /* server clock (seconds since server creation) */
+  double submit_s, admit_s, done_s; /* server clock (s since server creation); submit_s: entry to submit */
   int64_t pp;                /* pixel-pulse pairs of the request */
   int32_t rounds;            /* scheduling rounds the request took part in */
   int32_t preemptions;
   int32_t segments;          /* pulse segments per tile (1: the tiles' pulses run in order) */
+  double round_s;            /* start of the request's last scheduling round (work list build) */
+  double gpu_done_s;         /* its kernels of that round finished (then: result copy-out) */
 } sarbp_request_info;
 
 typedef struct sarbp_server_stats {
@@ -220,10 +222,12 @@ typedef struct sarbp_server_stats {
 } sarbp_server_stats;
 
 int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
-/* Copy the request's inputs and queue it.  out_c128 (2*n float64) and ncontrib (n int32,
- * nullable) receive the result when the request completes and must stay valid until then.
- * SARBP_EBUDGET if the request alone exceeds the budget.  Once a request is done or failed the
- * server keeps only its record (state, timings, counts): its copy of the inputs is released. */
+/* Copy the request's inputs (into page-locked host memory the server reuses) and queue it.
+ * out_c128 (2*n float64) and ncontrib (n int32, nullable) receive the result when the request
+ * completes and must stay valid until then.  SARBP_EBUDGET if the request alone exceeds the
+ * budget.  The request's latency (sarbp_request_info: done_s - submit_s) starts at the entry to
+ * this call.  Once a request is done or failed the server keeps only its record (state, timings,
+ * counts): its copy of the inputs is released. */
 int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
                         int32_t *ncontrib, int64_t *id);
 int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state);

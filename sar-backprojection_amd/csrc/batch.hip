@@ -61,7 +61,7 @@ void DevicePool::release() {
 }
 
 void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
-                   int64_t &pp) {
+                   int64_t &pp, SplitRows *rows) {
   const int tx = r.tile_x ? r.tile_x : default_tiling(r.precision).tile_x;
   const int64_t dimx = r.dimx > 0 ? r.dimx : (r.n > 0 ? r.n : 1);
   const int64_t dimy = r.n > 0 ? r.n / dimx : 0;
@@ -69,22 +69,52 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
   const int64_t nt = tg.ntiles();
   tile_len.assign((size_t)nt, 0);
   pp = 0;
+  if (rows) *rows = SplitRows();
   if (r.n <= 0) return;
-  std::vector<int> tlo((size_t)nt, INT_MAX), thi((size_t)nt, INT_MIN);
-  for (int64_t iy = 0; iy < dimy; ++iy)
-    for (int64_t ix = 0; ix < dimx; ++ix) {
-      const int64_t i = ix + dimx * iy;
-      if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
-      int64_t lo, hi;
-      sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, lo, hi);
-      if (hi <= lo) continue;
-      pp += hi - lo;
-      const int64_t t = (iy / tg.ty) * tg.ntx + ix / tg.tx;
-      tlo[(size_t)t] = std::min(tlo[(size_t)t], (int)lo);
-      thi[(size_t)t] = std::max(thi[(size_t)t], (int)hi);
+  const double *htx = ds->htx.data(), *hty = ds->hty.data(), *htz = ds->htz.data();
+  for (int64_t by = 0; by < tg.nty; ++by) {
+    const int64_t y0 = by * tg.ty, y1 = std::min(dimy, y0 + tg.ty);
+    for (int64_t bx = 0; bx < tg.ntx; ++bx) {
+      const int64_t x0 = bx * tg.tx, x1 = std::min(dimx, x0 + tg.tx);
+      int tlo = INT_MAX, thi = INT_MIN;
+      for (int64_t iy = y0; iy < y1; ++iy) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        int64_t ends[2][3] = {{-1, 0, 0}, {-1, 0, 0}};  // first and last pixel of the row: i, a0, a1
+        for (int64_t ix = x0; ix < x1; ++ix) {
+          const int64_t i = ix + dimx * iy;
+          if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
+          int64_t a0, a1;
+          sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, a0, a1);
+          if (a1 <= a0) continue;
+          pp += a1 - a0;
+          tlo = std::min(tlo, (int)a0);
+          thi = std::max(thi, (int)a1);
+          if (!rows) continue;
+          if (ends[0][0] < 0) ends[0][0] = i, ends[0][1] = a0, ends[0][2] = a1;
+          ends[1][0] = i, ends[1][1] = a0, ends[1][2] = a1;
+          const double p[3] = {r.x[i], r.y[i], r.z[i]};
+          for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p[k]);
+            hi[k] = std::max(hi[k], p[k]);
+          }
+        }
+        if (!rows || ends[0][0] < 0) continue;
+        // the along-track direction cosine varies smoothly along a row: taken at the aperture ends
+        // of the row's first and last pixels
+        double c2max = 0.0;
+        for (const auto &e : ends)
+          for (int64_t a : {e[1], e[2] - 1}) {
+            const double dx = r.x[e[0]] - htx[a], dy = r.y[e[0]] - hty[a], dz = r.z[e[0]] - htz[a];
+            c2max = std::max(c2max, dx * dx / (dx * dx + dy * dy + dz * dz));
+          }
+        const double ox = 0.5 * (hi[0] - lo[0]), oy = 0.5 * (hi[1] - lo[1]), oz = 0.5 * (hi[2] - lo[2]);
+        rows->any = true;
+        rows->amax = std::max(rows->amax, ox * std::sqrt(c2max) + oy + oz);
+        rows->omax = std::max(rows->omax, std::sqrt(ox * ox + oy * oy + oz * oz));
+      }
+      tile_len[(size_t)(by * tg.ntx + bx)] = thi > tlo ? thi - tlo : 0;
     }
-  for (int64_t t = 0; t < nt; ++t)
-    tile_len[(size_t)t] = thi[(size_t)t] > tlo[(size_t)t] ? thi[(size_t)t] - tlo[(size_t)t] : 0;
+  }
 }
 
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
@@ -192,20 +222,36 @@ BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
 }
 
 void build_queues(const std::vector<WorkItemCost> &work, std::vector<BatchItem> &items, std::vector<int32_t> &seg) {
-  // Longest first; then dealt to the queues in runs of K consecutive items, so each queue (XCD)
-  // works through neighbouring tiles and every queue sees the expensive items early.
-  std::vector<WorkItemCost> w(work);
-  std::stable_sort(w.begin(), w.end(), [](const WorkItemCost &a, const WorkItemCost &b) { return a.cost > b.cost; });
+  // Longest first (a counting sort over 32-pulse cost classes, list order within a class: the
+  // scheduling order only balances the load, results do not depend on it); then dealt to the
+  // queues in runs of K consecutive items, so each queue (XCD) works through neighbouring tiles and
+  // every queue sees the expensive items early.
+  constexpr int NB = 1024;
   constexpr size_t K = 8;
-  std::vector<std::vector<BatchItem>> q(SARBP_BATCH_QUEUES);
-  for (size_t k = 0; k < w.size(); ++k) q[(k / K) % SARBP_BATCH_QUEUES].push_back(w[k].item);
-  items.clear();
-  seg.assign(SARBP_BATCH_QUEUES + 1, 0);
-  for (int qi = 0; qi < SARBP_BATCH_QUEUES; ++qi) {
-    seg[qi] = (int32_t)items.size();
-    items.insert(items.end(), q[qi].begin(), q[qi].end());
+  auto cls = [](int64_t c) { return NB - 1 - (int)std::min<int64_t>(NB - 1, std::max<int64_t>(0, c) >> 5); };
+  const size_t n = work.size();
+  std::vector<int32_t> start(NB + 1, 0);
+  for (size_t k = 0; k < n; ++k) start[(size_t)cls(work[k].cost)]++;
+  int32_t acc = 0;
+  for (int c = 0; c <= NB; ++c) {
+    const int32_t m = c < NB ? start[(size_t)c] : 0;
+    start[(size_t)c] = acc;
+    acc += m;
   }
-  seg[SARBP_BATCH_QUEUES] = (int32_t)items.size();
+  std::vector<int32_t> order(n);
+  for (size_t k = 0; k < n; ++k) order[(size_t)start[(size_t)cls(work[k].cost)]++] = (int32_t)k;
+  size_t cnt[SARBP_BATCH_QUEUES] = {}, pos[SARBP_BATCH_QUEUES];
+  for (size_t k = 0; k < n; ++k) cnt[(k / K) % SARBP_BATCH_QUEUES]++;
+  seg.assign(SARBP_BATCH_QUEUES + 1, 0);
+  size_t o = 0;
+  for (int q = 0; q < SARBP_BATCH_QUEUES; ++q) {
+    seg[(size_t)q] = (int32_t)o;
+    pos[q] = o;
+    o += cnt[q];
+  }
+  seg[SARBP_BATCH_QUEUES] = (int32_t)o;
+  items.resize(n);
+  for (size_t k = 0; k < n; ++k) items[pos[(k / K) % SARBP_BATCH_QUEUES]++] = work[(size_t)order[k]].item;
 }
 
 int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
@@ -219,21 +265,35 @@ int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<
     if (e == hipSuccess) cap = need;
     return e;
   };
+  auto up = [](size_t v) { return (v + 255) / 256 * 256; };
   const size_t rb = std::max<size_t>(1, reqs.size()) * sizeof(BatchReqDev);
   const size_t ib = std::max<size_t>(1, items.size()) * sizeof(BatchItem);
+  const size_t sb = (SARBP_BATCH_QUEUES + 1) * sizeof(int32_t);
   hipError_t e = grow((void **)&d_reqs, cap_reqs, rb);
   if (e == hipSuccess) e = grow((void **)&d_items, cap_items, ib);
-  if (e == hipSuccess && !d_seg) e = hipMalloc((void **)&d_seg, (SARBP_BATCH_QUEUES + 1) * sizeof(int32_t));
-  if (e == hipSuccess && !d_heads) e = hipMalloc((void **)&d_heads, SARBP_BATCH_QUEUES * sizeof(int32_t));
+  if (e == hipSuccess && !d_seg) e = hipMalloc((void **)&d_seg, sb);
+  if (e == hipSuccess && !d_heads) e = hipMalloc((void **)&d_heads, SARBP_HEADS_BYTES);
+  const size_t stage = up(rb) + up(ib) + up(sb);
+  if (e == hipSuccess && stage > cap_stage) {  // page-locked staging: the uploads are plain DMA
+    const size_t want = std::max(stage, cap_stage * 2);
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    cap_stage = 0;
+    e = hipHostMalloc((void **)&h_stage, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap_stage = want;
+  }
   if (e != hipSuccess) {
     (void)hipGetLastError();
     return sarbp_fail(SARBP_ENOMEM, "work-list allocation failed");
   }
-  if (!reqs.empty()) e = hipMemcpyAsync(d_reqs, reqs.data(), reqs.size() * sizeof(BatchReqDev), hipMemcpyHostToDevice, st);
+  char *hr = h_stage, *hi = h_stage + up(rb), *hs = hi + up(ib);
+  if (!reqs.empty()) std::memcpy(hr, reqs.data(), reqs.size() * sizeof(BatchReqDev));
+  if (!items.empty()) std::memcpy(hi, items.data(), items.size() * sizeof(BatchItem));
+  std::memcpy(hs, seg.data(), seg.size() * sizeof(int32_t));
+  if (!reqs.empty()) e = hipMemcpyAsync(d_reqs, hr, reqs.size() * sizeof(BatchReqDev), hipMemcpyHostToDevice, st);
   if (e == hipSuccess && !items.empty())
-    e = hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(d_seg, seg.data(), seg.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    e = hipMemcpyAsync(d_items, hi, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_seg, hs, seg.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return sarbp_hip_fail(e, "work-list upload");
   nitems = (int)items.size();
   return SARBP_OK;
@@ -244,6 +304,7 @@ void ItemBuffers::release() {
   if (d_items) (void)hipFree(d_items);
   if (d_seg) (void)hipFree(d_seg);
   if (d_heads) (void)hipFree(d_heads);
+  if (h_stage) (void)hipHostFree(h_stage);
   *this = ItemBuffers();
 }
 
@@ -255,6 +316,7 @@ struct sarbp_batch {
   int built_waves = 0;
   int tile_px = 64, chunk = 32;
   int64_t total_px = 0;
+  bool any_f64 = false, any_split = false;
 };
 
 namespace {
@@ -290,8 +352,8 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   hipError_t e = hipEventRecord(ds->ev0, ds->stream);
   if (e != hipSuccess) return sarbp_hip_fail(e, "event record");
   if (b->ib.nitems > 0) {
-    rc = sarbp_launch_batch(ds->device, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, waves, chunk,
-                            b->max_rows, 0, ds->stream);
+    rc = sarbp_launch_batch(ds->device, b->ib.d_reqs, b->ib.d_items, b->ib.d_seg, b->ib.d_heads, b->ib.nitems,
+                            sarbp_batch_mode(b->any_f64, b->any_split), waves, chunk, b->max_rows, 0, ds->stream);
     if (rc) return rc;
   }
   e = hipEventRecord(ds->ev1, ds->stream);
@@ -330,6 +392,7 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
       return rc;
     }
     b->total_px += reqs[r].n;
+    (reqs[r].precision == SARBP_PREC_SPLIT ? b->any_split : b->any_f64) = true;
     if (reqs[r].precision == SARBP_PREC_SPLIT)  // only split tiles stage LDS records
       b->max_rows = std::max(b->max_rows, b->reqs[(size_t)r].tg.ty);
   }

@@ -38,9 +38,19 @@ struct ReqBuf {
   std::vector<int> tile_len;      // union length of each tile (pulses)
 };
 
-// Tile grid, per-tile union lengths (pulses) and pixel-pulse pairs of a request (host, integers).
+// Geometry of a request's tile rows that bounds the split form's range-offset error: for each
+// row of each tile (its pixels with an aperture), o = the half-extent of the pixels' coordinates
+// and c = the largest along-track direction cosine |dx| / R at the aperture ends of the row's
+// first and last pixels.
+struct SplitRows {
+  bool any = false;   // some row has a pixel with an aperture
+  double amax = 0.0;  // largest o_x * c + o_y + o_z over the rows (metres)
+  double omax = 0.0;  // largest |o| over the rows (metres)
+};
+// Tile grid, per-tile union lengths (pulses) and pixel-pulse pairs of a request, in one pass over
+// the pixels tile by tile (host); with `rows`, also the tile rows' geometry above.
 void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
-                   int64_t &pp);
+                   int64_t &pp, SplitRows *rows = nullptr);
 // Allocate and upload one request (asynchronously on st; the caller synchronises).  state_slots:
 // BpState entries per pixel (0: none; serving: 1 for the state between rounds, or one per pulse
 // segment).  pool: where the device block comes from (null: hipMalloc); tile_len: precomputed by
@@ -57,12 +67,16 @@ struct WorkItemCost {
 // Order items longest first and deal them to the per-XCD queues.
 void build_queues(const std::vector<WorkItemCost> &work, std::vector<BatchItem> &items, std::vector<int32_t> &seg);
 
-// Device copies of a work list (grown on demand, reused across launches).
+// Device copies of a work list (grown on demand, reused across launches), uploaded from a
+// page-locked staging block.  The staging block is rewritten by the next upload: the previous
+// upload must have completed by then (every caller synchronises each launch).
 struct ItemBuffers {
   BatchReqDev *d_reqs = nullptr;
   BatchItem *d_items = nullptr;
   int32_t *d_seg = nullptr, *d_heads = nullptr;
   size_t cap_reqs = 0, cap_items = 0;
+  char *h_stage = nullptr;
+  size_t cap_stage = 0;
   int nitems = 0;
   int upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
              const std::vector<int32_t> &seg, hipStream_t st);

@@ -9,6 +9,7 @@
 // the request's saved per-pixel state, and either writes the pixel means (last range of the tile)
 // or saves the state for a later launch.  The arithmetic per pixel is the same whatever the item
 // boundaries or the other requests in the launch, so a request's result does not depend on the mix.
+#include <algorithm>
 #include <climits>
 
 #include "bp_tile.h"
@@ -17,6 +18,9 @@
 
 namespace {
 
+// MODE: which requests a launch may hold (SARBP_BATCH_ANY, _F64 or _SPLIT); it only removes code
+// that the launch's requests cannot reach, so every item computes the same bits in every mode.
+template <int MODE>
 __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem &it, const double2 *tab,
                                            int64_t tile, int rec0, int C) {
   const BpGeom g = r.g;
@@ -42,7 +46,7 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
     ci = st.ci;
     cnt = st.cnt;
   }
-  bool split = r.precision == SARBP_PREC_SPLIT;
+  bool split = MODE != SARBP_BATCH_F64 && r.precision == SARBP_PREC_SPLIT;
   SplitTile s;
   if (split) {
     s = split_tile(g, r.tg, t);
@@ -88,23 +92,22 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
 }
 
 // C pulses of records per refill, up to max_rows rows per tile: C * max_rows records per wavefront.
-// Compiled for four wavefronts per SIMD (at most 128 VGPRs): the FP64 and split paths together
-// need 141 registers unbounded (three waves); five or six waves spill in the loops and are slower
-// (64 mixed requests: 29.0 ms at four waves, 32.6 ms at three, 41 ms at five).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bp_batch(const BatchReqDev *__restrict__ reqs,
-                                                  const BatchItem *__restrict__ items,
-                                                  const int32_t *__restrict__ seg, int32_t *heads, int C,
-                                                  int max_rows) {
+template <int MODE>
+__device__ __forceinline__ void batch_loop(const BatchReqDev *__restrict__ reqs, const BatchItem *__restrict__ items,
+                                           const int32_t *__restrict__ seg, int32_t *heads, int C, int max_rows) {
   __shared__ int32_t next_item;
   __shared__ double2 tab[SARBP_SCTAB];  // every request's table holds the same values
   load_sctab(reqs[0].g, tab);
   const int wave = threadIdx.x >> 6;
-  const int waves = blockDim.x >> 6;
   const int rec0 = wave * C * max_rows;  // this wavefront's records in sarbp_lds_rec
   const int home = blockIdx.x % SARBP_BATCH_QUEUES;
   for (int qi = 0; qi < SARBP_BATCH_QUEUES;) {
     const int q = (home + qi) % SARBP_BATCH_QUEUES;
-    if (threadIdx.x == 0) next_item = seg[q] + atomicAdd(&heads[q], 1);
+    if (threadIdx.x == 0) {  // heads only grow: a drained queue is seen without an atomic
+      int32_t *head = heads + q * SARBP_HEAD_STRIDE;
+      const int32_t seen = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      next_item = seg[q] + seen >= seg[q + 1] ? seg[q + 1] : seg[q] + atomicAdd(head, 1);
+    }
     __syncthreads();
     const int32_t k = next_item;
     __syncthreads();
@@ -115,9 +118,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     const BatchItem it = items[k];
     const BatchReqDev &r = reqs[it.req];
     const int64_t tile = (int64_t)it.tile_lo + wave;
-    if (tile < r.tg.ntiles()) batch_tile(r, it, tab, tile, rec0, C);
-    (void)waves;
+    if (tile < r.tg.ntiles()) batch_tile<MODE>(r, it, tab, tile, rec0, C);
   }
+}
+
+// Any mix of requests: compiled for four wavefronts per SIMD (at most 128 VGPRs).  The FP64 and
+// split paths together need 141 registers unbounded (three waves); five or six waves spill in the
+// loops and are slower (64 mixed requests: 29.0 ms at four waves, 32.6 ms at three, 41 ms at five).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_bp_batch(
+    const BatchReqDev *__restrict__ reqs, const BatchItem *__restrict__ items, const int32_t *__restrict__ seg,
+    int32_t *heads, int C, int max_rows) {
+  batch_loop<SARBP_BATCH_ANY>(reqs, items, seg, heads, C, max_rows);
+}
+
+// FP64 requests only: six wavefronts per SIMD (80 VGPRs; 32 FP64 requests 18.2 ms against 23.3 ms
+// in the mixed kernel; seven waves spill and take twice as long).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_bp_batch_f64(
+    const BatchReqDev *__restrict__ reqs, const BatchItem *__restrict__ items, const int32_t *__restrict__ seg,
+    int32_t *heads, int C, int max_rows) {
+  batch_loop<SARBP_BATCH_F64>(reqs, items, seg, heads, C, max_rows);
+}
+
+// Split requests only (with the FP64 terms of tiles that are not compact): five wavefronts per
+// SIMD (96 VGPRs; 32 split requests 8.1 ms, against 8.5 ms in the mixed kernel and 8.9 ms at six
+// waves, where the loops spill).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_bp_batch_split(
+    const BatchReqDev *__restrict__ reqs, const BatchItem *__restrict__ items, const int32_t *__restrict__ seg,
+    int32_t *heads, int C, int max_rows) {
+  batch_loop<SARBP_BATCH_SPLIT>(reqs, items, seg, heads, C, max_rows);
 }
 
 __global__ __launch_bounds__(256) void k_bp_combine(const BpState *__restrict__ part, int nseg, int64_t n,
@@ -152,23 +180,29 @@ size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
 }
 
 int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
-                       int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks, hipStream_t st) {
+                       int32_t *d_heads, int nitems, int mode, int waves, int chunk, int max_rows, int grid_blocks,
+                       hipStream_t st) {
+  if (mode != SARBP_BATCH_ANY && mode != SARBP_BATCH_F64 && mode != SARBP_BATCH_SPLIT)
+    return sarbp_fail(SARBP_EINVAL, "unknown batch launch mode");
+  auto kern = mode == SARBP_BATCH_F64 ? k_bp_batch_f64 : mode == SARBP_BATCH_SPLIT ? k_bp_batch_split : k_bp_batch;
   if (waves != 1 && waves != 2 && waves != 4) return sarbp_fail(SARBP_EINVAL, "waves per block must be 1, 2 or 4");
   if (chunk < 8 || chunk > 1024 || chunk % 8) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 8..1024, x8");
   if (max_rows < 1 || max_rows > 64) return sarbp_fail(SARBP_EINVAL, "bad tile row count");
   const size_t lds = sarbp_batch_lds_bytes(waves, chunk, max_rows);
   if (lds > 64 * 1024) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk x tile rows needs more than 64 KiB of LDS");
-  hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_BATCH_QUEUES * sizeof(int32_t), st);
+  if (nitems <= 0) return SARBP_OK;
+  hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_HEADS_BYTES, st);
   if (e != hipSuccess) return sarbp_hip_fail(e, "queue reset");
   if (grid_blocks <= 0) {
     int per_cu = 0, ncu = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bp_batch, waves * 64, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, waves * 64, lds);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e != hipSuccess) return sarbp_hip_fail(e, "occupancy query");
     if (per_cu < 1) per_cu = 1;
     grid_blocks = per_cu * ncu;
   }
-  hipLaunchKernelGGL(k_bp_batch, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, d_reqs, d_items, d_seg,
+  grid_blocks = std::min(grid_blocks, nitems);  // a block beyond the item count would find no work
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid_blocks), dim3(waves * 64), lds, st, d_reqs, d_items, d_seg,
                      d_heads, chunk, max_rows);
   e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "batch kernel launch");

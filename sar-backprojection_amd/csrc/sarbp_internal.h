@@ -208,13 +208,28 @@ struct BatchItem {
 };
 
 constexpr int SARBP_BATCH_QUEUES = 8;  // one work queue per XCD, with stealing
+// Queue heads sit 256 bytes apart (one per cache line and memory channel): every block of the
+// persistent grid visits every queue, and heads sharing one line would serialise all those
+// atomics in one place.
+constexpr int SARBP_HEAD_STRIDE = 64;   // int32 elements between queue heads
+constexpr size_t SARBP_HEADS_BYTES = (size_t)SARBP_BATCH_QUEUES * SARBP_HEAD_STRIDE * sizeof(int32_t);
+
+// Which requests one batch launch holds: any mix, FP64 requests only, or split requests only (the
+// last two run instantiations with fewer registers and more wavefronts per SIMD; same results).
+constexpr int SARBP_BATCH_ANY = 0, SARBP_BATCH_F64 = 1, SARBP_BATCH_SPLIT = 2;
+// The launch mode for a set of request precisions.
+inline int sarbp_batch_mode(bool any_f64, bool any_split) {
+  return any_f64 && any_split ? SARBP_BATCH_ANY : any_split ? SARBP_BATCH_SPLIT : SARBP_BATCH_F64;
+}
 
 // Launch the persistent batch kernel: `waves` wavefronts per block (1, 2 or 4), pulse chunk C,
 // tiles of at most max_rows rows (LDS holds C * max_rows pulse records per wavefront).
-// d_heads: SARBP_BATCH_QUEUES ints, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item
-// offsets.  grid_blocks <= 0 sizes the grid from the occupancy.
+// d_heads: SARBP_HEADS_BYTES, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item offsets;
+// nitems: the number of items; mode: SARBP_BATCH_* covering the requests' precisions.  grid_blocks <= 0 sizes the grid from the occupancy, at most one
+// block per item.
 int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
-                       int32_t *d_heads, int waves, int chunk, int max_rows, int grid_blocks, hipStream_t st);
+                       int32_t *d_heads, int nitems, int mode, int waves, int chunk, int max_rows, int grid_blocks,
+                       hipStream_t st);
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);
 // Pixel means from the partial sums of nseg pulse segments (part[k * n + i], added in the order
 // k = 0 .. nseg-1): out[i], ncnt[i] for i < n.

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
@@ -27,13 +28,62 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// Page-locked host blocks for the requests' pixel copies, reused across requests: fresh memory
+// per request costs page faults on the submit path, and uploads from pageable memory are staged
+// through the runtime's own buffers.  Same size classes and reuse rule as DevicePool; its own lock
+// (submit runs in the callers' threads, finish in the scheduling thread).
+struct PinnedPool {
+  std::mutex mu;
+  std::multimap<size_t, char *> blocks;
+  size_t cached = 0;
+  size_t limit = (size_t)1 << 30;
+  char *get(size_t bytes, size_t *cap) {
+    const size_t want = (bytes + ((size_t)1 << 20) - 1) >> 20 << 20;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = blocks.lower_bound(want);
+      if (it != blocks.end() && it->first <= 2 * want) {
+        char *p = it->second;
+        *cap = it->first;
+        cached -= it->first;
+        blocks.erase(it);
+        return p;
+      }
+    }
+    char *p = nullptr;
+    if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    *cap = want;
+    return p;
+  }
+  void put(char *p, size_t cap) {
+    if (!p) return;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (cached + cap <= limit) {
+        blocks.emplace(cap, p);
+        cached += cap;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
+  void release() {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto &b : blocks) (void)hipHostFree(b.second);
+    blocks.clear();
+    cached = 0;
+  }
+};
+
 struct ServerReq {
   int64_t id = 0;
   sarbp_dataset *ds = nullptr;
-  std::vector<double> x, y, z;
-  std::vector<uint8_t> mask;
-  std::vector<int32_t> index, sint;
-  sarbp_request req{};  // points into the host copies above
+  char *host = nullptr;  // page-locked copy of the pixel arrays (from the server's PinnedPool)
+  size_t host_cap = 0;
+  sarbp_request req{};  // points into the host copy above
   double *out = nullptr;
   int32_t *ncnt = nullptr;
   int32_t state = SARBP_REQ_QUEUED;
@@ -47,7 +97,7 @@ struct ServerReq {
   std::vector<BpState> saved;  // partial state of a preempted request
   std::vector<int> tile_len;   // per-tile union lengths, computed at submission (caller's thread)
   bool preempt = false;
-  double submit_s = 0, admit_s = -1, done_s = -1;
+  double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
   int32_t rounds = 0, preemptions = 0;
   int64_t pp = 0;
   int nseg = 1;     // pulse segments per tile (> 1: focused whole in its first round)
@@ -70,46 +120,16 @@ int segments_for(int64_t live_tiles, int max_len, int64_t target, int chunk, int
 }
 
 // A-priori RMS phase error (rad) of a request's split result against its FP64 result, over the
-// pixels with |a| > 1e-5 max|a|.  Per-term model: the FP32 error of the range offset
-// dR ~ 1.2e-7 x |a|, with |a| = |u.o| bounded per tile row from the row's offsets and the largest
-// along-track direction cosine over the pixels' apertures, plus ~2e-6 rad for the base phase and
-// the hardware sin/cos.  The sums' errors are partly correlated along the aperture and dark pixels
-// weigh in the RMS, so the measured RMS error is 0.2-2.0 times this model (72 requests on grids
-// 1-6 m apart, Sint 400-3500); the estimate is the model times 3.
-double split_phase_estimate(const sarbp_dataset *ds, const sarbp_request &r, int tile_x, const double *tx,
-                            const double *ty, const double *tz) {
-  if (r.n == 0) return 0.0;
-  const int64_t dimx = r.dimx > 0 ? r.dimx : r.n, dimy = r.n / dimx;
+// pixels with |a| > 1e-5 max|a|, from its tile rows' geometry (SplitRows).  Per-term model: the
+// FP32 error of the range offset dR ~ 1.2e-7 x |a|, with |a| = |u.o| <= o_x c + o_y + o_z, plus
+// ~2e-6 rad for the base phase and the hardware sin/cos.  The sums' errors are partly correlated
+// along the aperture and dark pixels weigh in the RMS, so the measured RMS error is 0.2-2.0 times
+// this model (72 requests on grids 1-6 m apart, Sint 400-3500); the estimate is the model times 3.
+double split_phase_estimate(const sarbp_dataset *ds, const SplitRows &rows) {
+  if (!rows.any) return 0.0;
+  if (rows.omax > 48.0) return 1e9;  // rows this wide take FP64 terms in the kernel anyway
   const double kabs = std::fabs(-4.0 * SARBP_PI32_F64 / ds->lambda_val);
-  double amax = 0.0;
-  for (int64_t iy = 0; iy < dimy; ++iy)
-    for (int64_t x0 = 0; x0 < dimx; x0 += tile_x) {
-      const int64_t x1 = std::min(dimx, x0 + tile_x);
-      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, cmax = 0.0;
-      bool any = false;
-      for (int64_t ix = x0; ix < x1; ++ix) {
-        const int64_t i = ix + dimx * iy;
-        if (r.mask[i] != 1 || r.sint[i] == -9999) continue;
-        int64_t a0, a1;
-        sarbp_pulse_range(r.index[i], r.sint[i], ds->naz, a0, a1);
-        if (a1 <= a0) continue;
-        any = true;
-        const double p[3] = {r.x[i], r.y[i], r.z[i]};
-        for (int k = 0; k < 3; ++k) {
-          lo[k] = std::min(lo[k], p[k]);
-          hi[k] = std::max(hi[k], p[k]);
-        }
-        for (int64_t a : {a0, a1 - 1}) {  // direction cosine along x at the aperture ends
-          const double dx = p[0] - tx[a], dy = p[1] - ty[a], dz = p[2] - tz[a];
-          cmax = std::max(cmax, std::fabs(dx) / std::sqrt(dx * dx + dy * dy + dz * dz));
-        }
-      }
-      if (!any) continue;
-      const double ox = 0.5 * (hi[0] - lo[0]), oy = 0.5 * (hi[1] - lo[1]), oz = 0.5 * (hi[2] - lo[2]);
-      amax = std::max(amax, ox * cmax + oy + oz);
-      if (std::sqrt(ox * ox + oy * oy + oz * oz) > 48.0) return 1e9;  // kernel takes FP64 terms anyway
-    }
-  return 3.0 * (kabs * 1.2e-7 * amax + 2e-6);
+  return 3.0 * (kabs * 1.2e-7 * rows.amax + 2e-6);
 }
 
 }  // namespace
@@ -130,7 +150,12 @@ struct sarbp_server {
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ItemBuffers ib;
-  DevicePool pool;  // request blocks (scheduling thread only)
+  std::vector<BatchReqDev> devs;  // one round's work list (scheduling thread only, reused)
+  std::vector<WorkItemCost> work;
+  std::vector<BatchItem> items;
+  std::vector<int32_t> seg;
+  DevicePool pool;    // request blocks (scheduling thread only)
+  PinnedPool hpool;   // host copies of the requests' pixels
   int64_t bytes_in_use = 0;
   double rate = 3e11;  // pixel-pulse pairs per second, running estimate
   Clock::time_point t0 = Clock::now();
@@ -147,14 +172,11 @@ struct sarbp_server {
   // A finished (done or failed) request keeps only its record: its host copy of the pixel arrays
   // and any saved state are dropped, so a long-running server holds the pixels of the requests in
   // flight only.
-  static void finish(ServerReq *r, int32_t state) {
+  void finish(ServerReq *r, int32_t state) {
     r->state = state;
-    std::vector<double>().swap(r->x);
-    std::vector<double>().swap(r->y);
-    std::vector<double>().swap(r->z);
-    std::vector<uint8_t>().swap(r->mask);
-    std::vector<int32_t>().swap(r->index);
-    std::vector<int32_t>().swap(r->sint);
+    hpool.put(r->host, r->host_cap);  // no upload from it is pending: the request left the device
+    r->host = nullptr;
+    r->host_cap = 0;
     std::vector<BpState>().swap(r->saved);
     std::vector<int>().swap(r->tile_len);
     r->req.x = r->req.y = r->req.z = nullptr;
@@ -258,13 +280,17 @@ void sarbp_server::run() {
     int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)std::max<int64_t>(1, live_px)));
     span = std::max(chunk, (span + chunk - 1) / chunk * chunk);
     std::vector<ServerReq *> round(active);
-    std::vector<BatchReqDev> devs;
-    std::vector<WorkItemCost> work;
+    const double t_round = now();
+    for (ServerReq *r : round) r->round_s = t_round;
+    devs.clear();
+    work.clear();
     int max_rows = 1;
     int64_t round_pp = 0;
+    bool any_f64 = false, any_split = false;
     for (size_t k = 0; k < round.size(); ++k) {
       ServerReq *r = round[k];
       devs.push_back(reqbuf_dev(r->ds, r->buf));
+      (r->precision == SARBP_PREC_SPLIT ? any_split : any_f64) = true;
       if (r->precision == SARBP_PREC_SPLIT) max_rows = std::max(max_rows, r->buf.tg.ty);
       const int64_t nt = r->buf.tg.ntiles();
       if (r->nseg > 1) {  // every segment of every tile, this round
@@ -288,8 +314,10 @@ void sarbp_server::run() {
         round_pp += (int64_t)c * 64;
       }
     }
-    std::vector<BatchItem> items;
-    std::vector<int32_t> seg;
+    // requests that complete in this round: their results are copied out in the same stream pass
+    std::vector<ServerReq *> fin;
+    for (ServerReq *r : round)
+      if (r->nseg > 1 || r->off + span >= r->max_len) fin.push_back(r);
     build_queues(work, items, seg);
     lk.unlock();
     float ms = 0.f;
@@ -298,24 +326,42 @@ void sarbp_server::run() {
     if (!rc && !items.empty()) {
       e = hipEventRecord(e0, st);
       if (e == hipSuccess)
-        rc = sarbp_launch_batch(device, ib.d_reqs, ib.d_items, ib.d_seg, ib.d_heads, 1, chunk, max_rows, 0, st);
+        rc = sarbp_launch_batch(device, ib.d_reqs, ib.d_items, ib.d_seg, ib.d_heads, (int)items.size(),
+                                sarbp_batch_mode(any_f64, any_split), 1, chunk, max_rows, 0, st);
       for (ServerReq *r : round)  // segmented requests: means from the segments' partial sums
         if (!rc && e == hipSuccess && r->nseg > 1)
           rc = sarbp_launch_combine(r->buf.state, r->nseg, r->buf.n, r->buf.out, r->buf.ncnt, st);
       if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
-      if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
-      if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    // results of completing requests into their page-locked blocks (the pixel copy is no longer
+    // needed), then one wait for the whole round
+    for (ServerReq *r : fin) {
+      if (rc || e != hipSuccess || !r->buf.n || !r->host) continue;
+      const size_t n = (size_t)r->buf.n;
+      e = hipMemcpyAsync(r->host, r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(r->host + (n * sizeof(double2) + 255) / 256 * 256, r->buf.ncnt, n * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, st);
+    }
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(st);
+    if (!rc && e == hipSuccess && !items.empty()) e = hipEventElapsedTime(&ms, e0, e1);
+    const bool failed = rc || e != hipSuccess;
+    if (failed) {
+      (void)hipStreamSynchronize(st);  // nothing of this round may still run when its blocks return to the pool
+    } else {
+      for (ServerReq *r : fin) {  // into the caller's buffers
+        if (!r->buf.n || !r->host) continue;
+        const size_t n = (size_t)r->buf.n;
+        if (r->out) std::memcpy(r->out, r->host, n * sizeof(double2));
+        if (r->ncnt) std::memcpy(r->ncnt, r->host + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
+      }
     }
     lk.lock();
+    const double t_gpu = now();
+    for (ServerReq *r : round) r->gpu_s = t_gpu;
     stats.rounds++;
     stats.gpu_busy_s += ms * 1e-3;
     if (ms > 0.05f && round_pp > 0) rate = 0.7 * rate + 0.3 * (round_pp * 0.8 / (ms * 1e-3));  // lanes ~80 % active
-    const bool failed = rc || e != hipSuccess;
-    if (failed) {  // nothing of this round may still run when its blocks return to the pool
-      lk.unlock();
-      (void)hipStreamSynchronize(st);
-      lk.lock();
-    }
     for (ServerReq *r : round) {
       r->rounds++;
       if (failed) {
@@ -325,25 +371,12 @@ void sarbp_server::run() {
         continue;
       }
       r->off = r->nseg > 1 ? r->max_len : r->off + span;
-      if (r->off >= r->max_len) {  // every tile wrote its means: copy the result out
-        lk.unlock();
-        hipError_t ce = hipSuccess;
-        if (r->out && r->buf.n)
-          ce = hipMemcpyAsync(r->out, r->buf.out, (size_t)r->buf.n * sizeof(double2), hipMemcpyDeviceToHost, st);
-        if (ce == hipSuccess && r->ncnt && r->buf.n)
-          ce = hipMemcpyAsync(r->ncnt, r->buf.ncnt, (size_t)r->buf.n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
-        if (ce == hipSuccess) ce = hipStreamSynchronize(st);
-        lk.lock();
+      if (r->off >= r->max_len) {  // every tile wrote its means, copied out above
         release(r);
         r->done_s = now();
-        if (ce == hipSuccess) {
-          finish(r, SARBP_REQ_DONE);
-          stats.completed++;
-          stats.pp_done += r->pp;
-        } else {
-          finish(r, SARBP_REQ_FAILED);
-          stats.failed++;
-        }
+        finish(r, SARBP_REQ_DONE);
+        stats.completed++;
+        stats.pp_done += r->pp;
       }
     }
     active.erase(std::remove_if(active.begin(), active.end(),
@@ -410,6 +443,7 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
 int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
                         int32_t *ncontrib, int64_t *id) {
   if (!s || !ds || !req || !id) return sarbp_fail(SARBP_EINVAL, "null argument");
+  const double t_entry = s->now();  // a request's latency includes its submission
   if (req->n < 0 || (req->n > 0 && (!req->x || !req->y || !req->z || !req->mask || !req->index || !req->sint)))
     return sarbp_fail(SARBP_EINVAL, "bad request pixel arrays");
   if (req->n > 0 && !out_c128) return sarbp_fail(SARBP_EINVAL, "output buffer is null");
@@ -417,36 +451,29 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
     return sarbp_fail(SARBP_EINVAL, "request pixel count must be a multiple of dimx");
   if (req->precision != SARBP_PREC_F64 && req->precision != SARBP_PREC_SPLIT)
     return sarbp_fail(SARBP_EINVAL, "unknown request precision");
+  const int tx = req->tile_x ? req->tile_x : default_tiling(req->precision).tile_x;
+  if (tx != 64 && tx != 32 && tx != 16 && tx != 8 && tx != 4) return sarbp_fail(SARBP_EINVAL, "bad tile_x");
   auto r = std::make_unique<ServerReq>();
-  const size_t n = (size_t)req->n;
-  r->x.assign(req->x, req->x + n);
-  r->y.assign(req->y, req->y + n);
-  r->z.assign(req->z, req->z + n);
-  r->mask.assign(req->mask, req->mask + n);
-  r->index.assign(req->index, req->index + n);
-  r->sint.assign(req->sint, req->sint + n);
-  r->req = *req;
-  r->req.x = r->x.data();
-  r->req.y = r->y.data();
-  r->req.z = r->z.data();
-  r->req.mask = r->mask.data();
-  r->req.index = r->index.data();
-  r->req.sint = r->sint.data();
   r->ds = ds;
   r->out = out_c128;
   r->ncnt = ncontrib;
-  const int tx = req->tile_x ? req->tile_x : default_tiling(req->precision).tile_x;
-  if (tx != 64 && tx != 32 && tx != 16 && tx != 8 && tx != 4) return sarbp_fail(SARBP_EINVAL, "bad tile_x");
+  r->req = *req;  // the scheduling decisions below read the caller's arrays
   r->precision = req->precision;
-  if (req->precision == SARBP_PREC_SPLIT) {
-    r->est = split_phase_estimate(ds, r->req, tx, ds->htx.data(), ds->hty.data(), ds->htz.data());
-    if (r->est > s->cfg.phase_err_limit_rad) r->precision = SARBP_PREC_F64;
-  }
-  r->req.precision = r->precision;
-  r->req.tile_x = r->precision == req->precision ? tx : 0;
+  r->req.tile_x = tx;
   {
     TileGrid tg;
-    request_tiles(ds, r->req, tg, r->tile_len, r->pp);
+    SplitRows rows;
+    const bool split = req->precision == SARBP_PREC_SPLIT;
+    request_tiles(ds, r->req, tg, r->tile_len, r->pp, split ? &rows : nullptr);
+    if (split) {
+      r->est = split_phase_estimate(ds, rows);
+      if (r->est > s->cfg.phase_err_limit_rad) {  // FP64 terms, on the FP64 tiling
+        r->precision = SARBP_PREC_F64;
+        r->req.precision = SARBP_PREC_F64;
+        r->req.tile_x = 0;
+        request_tiles(ds, r->req, tg, r->tile_len, r->pp);
+      }
+    }
   }
   {
     int max_len = 0;
@@ -464,10 +491,35 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) +
             up(nn * (int64_t)sizeof(BpState) * r->nseg);
   if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
+  // the request's own copy of the pixels, uploaded at admission: x y z (f64), index sint (i32),
+  // mask (u8), each 256-byte aligned
+  const size_t n = (size_t)nn;
+  if (n) {
+    const size_t o1 = (size_t)up(nn * 8), o2 = 2 * o1, o3 = 3 * o1, o4 = o3 + (size_t)up(nn * 4),
+                 o5 = o4 + (size_t)up(nn * 4);
+    r->host = s->hpool.get(o5 + n, &r->host_cap);
+    if (!r->host) return sarbp_fail(SARBP_ENOMEM, "page-locked host allocation failed");
+    char *h = r->host;
+    std::memcpy(h, req->x, n * 8);
+    std::memcpy(h + o1, req->y, n * 8);
+    std::memcpy(h + o2, req->z, n * 8);
+    std::memcpy(h + o3, req->index, n * 4);
+    std::memcpy(h + o4, req->sint, n * 4);
+    std::memcpy(h + o5, req->mask, n);
+    r->req.x = (const double *)h;
+    r->req.y = (const double *)(h + o1);
+    r->req.z = (const double *)(h + o2);
+    r->req.index = (const int32_t *)(h + o3);
+    r->req.sint = (const int32_t *)(h + o4);
+    r->req.mask = (const uint8_t *)(h + o5);
+  }
   std::lock_guard<std::mutex> lk(s->mu);
-  if (s->stop) return sarbp_fail(SARBP_ESTATE, "server is shutting down");
+  if (s->stop) {
+    s->hpool.put(r->host, r->host_cap);
+    return sarbp_fail(SARBP_ESTATE, "server is shutting down");
+  }
   r->id = s->next_id++;
-  r->submit_s = s->now();
+  r->submit_s = t_entry;
   *id = r->id;
   s->queue.push_back(r.get());
   s->reqs[r->id] = std::move(r);
@@ -550,6 +602,8 @@ int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info) {
   info->rounds = r->rounds;
   info->preemptions = r->preemptions;
   info->segments = r->nseg;
+  info->round_s = r->round_s;
+  info->gpu_done_s = r->gpu_s;
   return SARBP_OK;
 }
 
@@ -574,6 +628,9 @@ int sarbp_server_destroy(sarbp_server *s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   s->ib.release();
   s->pool.release();
+  for (auto &kv : s->reqs)  // preempted requests never resumed still hold their host copy
+    if (kv.second->host) (void)hipHostFree(kv.second->host);
+  s->hpool.release();
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
   if (s->st) (void)hipStreamDestroy(s->st);

@@ -54,7 +54,7 @@ class RequestInfo(ctypes.Structure):
     """sarbp_request_info (include/sarbp.h)."""
     _fields_ = [("state", _i32), ("precision", _i32), ("phase_err_est_rad", _d), ("submit_s", _d),
                 ("admit_s", _d), ("done_s", _d), ("pp", _i64), ("rounds", _i32), ("preemptions", _i32),
-                ("segments", _i32)]
+                ("segments", _i32), ("round_s", _d), ("gpu_done_s", _d)]
 
 
 class ServerStats(ctypes.Structure):

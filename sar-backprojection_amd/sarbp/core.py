@@ -109,10 +109,9 @@ class Dataset:
             self._h = None
 
     def __del__(self):
-        if not _lib.alive():
-            return
-        try:
-            self.close()
+        try:  # at interpreter exit the module's globals may already be gone
+            if _lib.alive():
+                self.close()
         except Exception:
             pass
 
@@ -207,10 +206,9 @@ class Scene:
             self._h = None
 
     def __del__(self):
-        if not _lib.alive():
-            return
-        try:
-            self.close()
+        try:  # at interpreter exit the module's globals may already be gone
+            if _lib.alive():
+                self.close()
         except Exception:
             pass
 

@@ -98,10 +98,9 @@ class Server:
             self._h = None
 
     def __del__(self):
-        if not _lib.alive():
-            return
-        try:
-            self.close()
+        try:  # at interpreter exit the module's globals may already be gone
+            if _lib.alive():
+                self.close()
         except Exception:
             pass
 
