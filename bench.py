@@ -231,6 +231,7 @@ def main():
         srv.close()
         extra["serving"] = {k: sv[k] for k in ("rate_req_per_s", "duration_s", "requests", "latency_p50_ms",
                                               "latency_p99_ms", "latency_max_ms", "pp_per_s", "precision_counts")}
+        extra["serving"]["latency"] = "entry to sarbp_server_submit -> result in the caller's buffer (queueing included)"
         extra["latency_p50_ms"] = sv["latency_p50_ms"]
         extra["latency_p99_ms"] = sv["latency_p99_ms"]
         log(f"serving {args.serve_rate}/s for {args.serve_seconds}s: p50 {sv['latency_p50_ms']:.2f} ms, "

@@ -112,7 +112,8 @@ def percentile(v, q):
 def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: float, seed: int = 0,
                 warmup_s: float = 2.0, log=None):
     """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at Poisson times of
-    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival (server clock).
+    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival on the server
+    clock, arrival being the entry to the library's submit call (queueing included).
     Finished requests are harvested as the run goes (their result buffers released), so a long run
     holds only the requests in flight."""
     rng = np.random.default_rng(seed)
