@@ -251,6 +251,16 @@ int sarbp_server_destroy(sarbp_server *s);
 int64_t sarbp_count_pp(const uint8_t *mask, const int32_t *index, const int32_t *sint, int64_t n,
                        int64_t naz);
 
+/*
+ * A-priori estimate of the split path's RMS phase error (rad) for a request's pixels on a dataset,
+ * against the FP64 result over the pixels with |a| > 1e-5 max|a|: the estimate behind the server's
+ * FP64 fallback (measured errors stay below it).  Host-side, from the split tiling's rows (the
+ * request's tile_x if its precision is SARBP_PREC_SPLIT, else the default); 1e9 when those rows are
+ * too wide for the split form (the kernel takes FP64 terms there), 0 without contributing pixels.
+ * No counterpart in the reference; for callers of sarbp_focus choosing a precision.
+ */
+int sarbp_split_error_estimate(sarbp_dataset *ds, const sarbp_request *req, double *rms_phase_rad);
+
 #ifdef __cplusplus
 }
 #endif

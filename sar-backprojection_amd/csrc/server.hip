@@ -528,6 +528,26 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   return SARBP_OK;
 }
 
+int sarbp_split_error_estimate(sarbp_dataset *ds, const sarbp_request *req, double *rms_phase_rad) {
+  if (!ds || !req || !rms_phase_rad) return sarbp_fail(SARBP_EINVAL, "null argument");
+  if (req->n < 0 || (req->n > 0 && (!req->x || !req->y || !req->z || !req->mask || !req->index || !req->sint)))
+    return sarbp_fail(SARBP_EINVAL, "bad request pixel arrays");
+  if (req->dimx < 0 || (req->n > 0 && req->dimx > 0 && req->n % req->dimx))
+    return sarbp_fail(SARBP_EINVAL, "request pixel count must be a multiple of dimx");
+  sarbp_request q = *req;
+  q.tile_x = req->precision == SARBP_PREC_SPLIT ? req->tile_x : 0;
+  q.precision = SARBP_PREC_SPLIT;
+  const int tx = q.tile_x ? q.tile_x : default_tiling(SARBP_PREC_SPLIT).tile_x;
+  if (tx != 64 && tx != 32 && tx != 16 && tx != 8 && tx != 4) return sarbp_fail(SARBP_EINVAL, "bad tile_x");
+  TileGrid tg;
+  std::vector<int> tile_len;
+  int64_t pp = 0;
+  SplitRows rows;
+  request_tiles(ds, q, tg, tile_len, pp, &rows);
+  *rms_phase_rad = split_phase_estimate(ds, rows);
+  return SARBP_OK;
+}
+
 int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state) {
   if (!s || !state) return sarbp_fail(SARBP_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);

@@ -139,6 +139,23 @@ class Dataset:
                                       out.ctypes.data, cnt.ctypes.data), "sarbp_focus")
         return (out, cnt) if with_counts else out
 
+    def split_error_estimate(self, x, y, z, mask, index, sint, dimx=0, tile_x=0) -> float:
+        """A-priori RMS phase error (rad) of the split path for these pixels (F-order grid of fast
+        dimension dimx, 0 = plain list) against FP64, over pixels above 1e-5 x max amplitude; the
+        estimate the server compares with its fallback limit (1e9: rows too wide for the split form)."""
+        c = lambda a, dt: np.ascontiguousarray(np.asarray(a, dtype=dt).ravel())  # noqa: E731
+        x, y, z = c(x, np.float64), c(y, np.float64), c(z, np.float64)
+        mask, index, sint = c(mask, np.uint8), c(index, np.int32), c(sint, np.int32)
+        n = x.size
+        if not all(a.size == n for a in (y, z, mask, index, sint)):
+            raise ValueError("pixel arrays must all have the same size")
+        req = _lib.Request(x.ctypes.data, y.ctypes.data, z.ctypes.data, mask.ctypes.data, index.ctypes.data,
+                           sint.ctypes.data, n, int(dimx), PREC_SPLIT, int(tile_x))
+        est = ctypes.c_double()
+        check(_lib.load().sarbp_split_error_estimate(self.handle, ctypes.byref(req), ctypes.byref(est)),
+              "sarbp_split_error_estimate")
+        return float(est.value)
+
     def eval_pairs(self, x, y, z, pulse) -> dict:
         """The FP64 path's per-(pixel, pulse) arithmetic on the device (sarbp_eval_pairs): R2, R,
         bin (-1 outside [0, Nra-1)), phase, sin, cos for equal-length x, y, z and pulse arrays."""

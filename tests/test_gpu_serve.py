@@ -167,6 +167,7 @@ def test_split_falls_back_to_fp64_when_estimate_exceeds_limit(gpu, world):
     i1, i2 = h1.info(), h2.info()
     print("estimates", i1["phase_err_est_rad"], i2["phase_err_est_rad"], flush=True)
     assert i1["precision"] == _lib.PREC_F64 and i1["phase_err_est_rad"] > 1e-3
+    assert ds.split_error_estimate(req.x, req.y, req.z, req.mask, req.index, req.sint, dimx=nx) == i1["phase_err_est_rad"]
     assert i2["precision"] == _lib.PREC_SPLIT and i2["phase_err_est_rad"] < 1e-3
     srv.close()
 
@@ -189,6 +190,7 @@ def test_split_estimate_bounds_measured_phase_error(gpu, world):
         h2 = srv.submit(ds, Request(*args, dimx=nx, precision="f64"))
         assert h1.wait(60) and h2.wait(60)
         est = h1.info()["phase_err_est_rad"]
+        assert est == ds.split_error_estimate(*args, dimx=nx)  # the same estimate for one-shot callers
         assert h1.info()["precision"] == _lib.PREC_SPLIT
         e = metrics.split_error_report(h1.result(), h2.result())
         print("step", step, "estimate", est, "measured", e, flush=True)
