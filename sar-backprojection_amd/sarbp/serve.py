@@ -5,6 +5,7 @@ arrivals of sub-scene requests and reports p50/p99 latency (completion - arrival
 included) and aggregate pixel-pulse pairs per second.
 
     python -m sarbp.serve --rates 5,20,50 --duration 60
+    python -m sarbp.serve --sweep          # isolated requests by size: latency phases
 """
 from __future__ import annotations
 
@@ -169,6 +170,33 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
     }
 
 
+def size_sweep(server: Server, ds, scene, sizes=((8, 8), (128, 64), (256, 128), (600, 144), (1024, 250)),
+               precisions=("split", "f64"), reps=4, x0=1000, y0=0):
+    """Latency of isolated requests by size: each (width, height) patch of ``scene`` at (x0, y0) is
+    submitted ``reps`` times, one at a time; the last repetition's phases are reported (ms):
+    submission to admission, admission to its round, the round (work list, kernels, result copies),
+    and the whole latency."""
+    rows = []
+    nx, ny = scene.shape
+    for (w, h) in sizes:
+        w, h = min(w, nx - x0), min(h, ny - y0)
+        sub = scene.sub(x0, x0 + w, y0, y0 + h)
+        f = lambda a: np.asarray(a).flatten(order="F")  # noqa: E731
+        for prec in precisions:
+            req = Request(f(sub.x), f(sub.y), f(sub.z), f(sub.mask), f(sub.index), f(sub.sint), dimx=w,
+                          precision=prec)
+            for _ in range(reps):
+                hd = server.submit(ds, req)
+                hd.wait()
+                i = hd.info()
+                time.sleep(0.002)
+            ms = lambda a, b: (i[b] - i[a]) * 1e3  # noqa: E731
+            rows.append({"w": w, "h": h, "precision": prec, "pp": i["pp"], "segments": i["segments"],
+                         "submit_to_admit_ms": ms("submit_s", "admit_s"), "admit_to_round_ms": ms("admit_s", "round_s"),
+                         "round_ms": ms("round_s", "gpu_done_s"), "latency_ms": ms("submit_s", "done_s")})
+    return rows
+
+
 def make_pool(scenes, nreq_per_dataset=32, seed=0):
     """Sub-scene requests (width 128-1024, height 32-250, Sint 500-4000, split:FP64 = 1:1)."""
     from .batch import patch_requests
@@ -191,12 +219,23 @@ def main(argv=None):
     ap.add_argument("--max-segments", type=int, default=0, help="pulse segments per tile of small requests "
                     "(0: library default; 1: off)")
     ap.add_argument("--small", action="store_true")
+    ap.add_argument("--sweep", action="store_true", help="latency of isolated requests by size instead")
     a = ap.parse_args(argv)
     from . import synth
     from .core import Dataset
     log = lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True)  # noqa: E731
     base = synth.full_scene(seed=1, **({"nx": 512, "ny": 250, "naz": 5121, "ntarget": 200} if a.small else {}))
     log(f"scene {base.name} ready")
+    if a.sweep:
+        ds = Dataset(base.raw, base.tx, base.ty, base.tz, base.near, base.dra, base.lam)
+        srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms, max_segments=a.max_segments)
+        rows = size_sweep(srv, ds, base, x0=min(1000, base.shape[0] // 4))
+        srv.close()
+        ds.close()
+        for r in rows:
+            log(json.dumps(r))
+        print(json.dumps({"sweep": rows}))
+        return
     datasets, scenes = [], []
     for k in range(a.datasets):
         raw = base.raw if k == 0 else np.asfortranarray(base.raw * np.complex64(np.exp(1j * 0.7 * k)))
