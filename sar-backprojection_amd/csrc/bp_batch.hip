@@ -179,6 +179,17 @@ size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
   return (size_t)waves * (size_t)chunk * (size_t)max_rows * sizeof(PulseRec);
 }
 
+int sarbp_batch_resident_blocks(int device, int mode, int waves, int chunk, int max_rows, int *blocks) {
+  auto kern = mode == SARBP_BATCH_F64 ? k_bp_batch_f64 : mode == SARBP_BATCH_SPLIT ? k_bp_batch_split : k_bp_batch;
+  int per_cu = 0, ncu = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, waves * 64,
+                                                              sarbp_batch_lds_bytes(waves, chunk, max_rows));
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return sarbp_hip_fail(e, "occupancy query");
+  *blocks = std::max(1, per_cu) * ncu;
+  return SARBP_OK;
+}
+
 int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d_items, const int32_t *d_seg,
                        int32_t *d_heads, int nitems, int mode, int waves, int chunk, int max_rows, int grid_blocks,
                        hipStream_t st) {

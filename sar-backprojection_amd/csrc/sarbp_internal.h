@@ -231,6 +231,8 @@ int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d
                        int32_t *d_heads, int nitems, int mode, int waves, int chunk, int max_rows, int grid_blocks,
                        hipStream_t st);
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);
+// Blocks of `waves` wavefronts of the batch kernel for `mode` that the device holds at once.
+int sarbp_batch_resident_blocks(int device, int mode, int waves, int chunk, int max_rows, int *blocks);
 // Pixel means from the partial sums of nseg pulse segments (part[k * n + i], added in the order
 // k = 0 .. nseg-1): out[i], ncnt[i] for i < n.
 int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out, int32_t *ncnt, hipStream_t st);

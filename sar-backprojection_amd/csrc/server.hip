@@ -104,14 +104,23 @@ struct ServerReq {
   int seg_len = 0;  // pulses per segment (a multiple of the chunk)
 };
 
-// Pulse segments per tile for a request whose live tiles (union length > 0) leave most of the
-// device idle: about `target` wavefronts in all, at most max_seg segments, each at least four
-// pulse chunks long.  Depends only on the request, the device and the configuration.
-int segments_for(int64_t live_tiles, int max_len, int64_t target, int chunk, int max_seg, int *seg_len) {
+// Pulse segments per tile for a request of `live` tiles (union length > 0) on a device that holds
+// `slots` wavefronts of the request's kernel at once.  The resident wavefronts take the items
+// (tiles x segments) in turns, so the launch lasts about ceil(live K / slots) turns of max_len / K
+// pulses; K is the smallest count within 5 % of the least such time, at most max_seg, with
+// segments at least four pulse chunks long.  A request of more than 4 x slots tiles keeps one
+// sequence per tile (and runs over several rounds).  Depends only on the request, the device and
+// the configuration.
+int segments_for(int64_t live, int max_len, int64_t slots, int chunk, int max_seg, int *seg_len) {
   *seg_len = 0;
-  if (live_tiles <= 0 || max_seg <= 1) return 1;
-  int64_t k = std::min<int64_t>(max_seg, target / live_tiles);
-  k = std::min<int64_t>(k, max_len / (4 * chunk));
+  if (live <= 0 || max_seg <= 1 || slots <= 0 || live > 4 * slots) return 1;
+  const int64_t kmax = std::min<int64_t>(max_seg, max_len / (4 * chunk));
+  if (kmax <= 1) return 1;
+  auto turns = [&](int64_t k) { return (double)((live * k + slots - 1) / slots) / (double)k; };
+  double best = turns(1);
+  for (int64_t k = 2; k <= kmax; ++k) best = std::min(best, turns(k));
+  int64_t k = 1;
+  while (turns(k) > 1.05 * best) ++k;
   if (k <= 1) return 1;
   const int64_t per = (max_len + k - 1) / k;
   const int len = (int)((per + chunk - 1) / chunk * chunk);
@@ -146,7 +155,7 @@ struct sarbp_server {
   bool stop = false;
   bool started = false;
   int device = 0;
-  int64_t target_waves = 8192;  // wavefronts that fill the device (32 per CU)
+  int64_t slots_f64 = 4096, slots_split = 4096;  // resident wavefronts of the FP64 / split batch kernels
   hipStream_t st = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   ItemBuffers ib;
@@ -418,7 +427,14 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
   hipError_t e = hipGetDevice(&s->device);
   int ncu = 0;
   if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, s->device);
-  if (e == hipSuccess && ncu > 0) s->target_waves = (int64_t)ncu * 32;
+  if (e == hipSuccess && ncu > 0) {
+    s->slots_f64 = s->slots_split = (int64_t)ncu * 16;
+    int b = 0;
+    if (sarbp_batch_resident_blocks(s->device, SARBP_BATCH_F64, 1, c.pulse_chunk, 1, &b) == SARBP_OK && b > 0)
+      s->slots_f64 = b;
+    if (sarbp_batch_resident_blocks(s->device, SARBP_BATCH_SPLIT, 1, c.pulse_chunk, 4, &b) == SARBP_OK && b > 0)
+      s->slots_split = b;
+  }
   if (e == hipSuccess) e = hipStreamCreate(&s->st);
   if (e == hipSuccess) e = hipEventCreate(&s->e0);
   if (e == hipSuccess) e = hipEventCreate(&s->e1);
@@ -482,7 +498,8 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
       max_len = std::max(max_len, v);
       live += v > 0;
     }
-    r->nseg = segments_for(live, max_len, s->target_waves, s->cfg.pulse_chunk, s->cfg.max_segments, &r->seg_len);
+    const int64_t slots = r->precision == SARBP_PREC_SPLIT ? s->slots_split : s->slots_f64;
+    r->nseg = segments_for(live, max_len, slots, s->cfg.pulse_chunk, s->cfg.max_segments, &r->seg_len);
   }
   // device bytes while admitted: pixels + outputs + state or segment slots (each array 256-byte
   // aligned)
