@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "batch_internal.h"
-#include "sched.h"
+#include "serve_policy.h"
 
 namespace {
 

@@ -1,4 +1,4 @@
-"""The serving loop's segment-count rule (csrc/sched.h), compiled on its own with the host compiler.
+"""The serving loop's segment-count rule (csrc/serve_policy.h), compiled on its own with the host compiler.
 
 For a request of `live` tiles whose longest tile union is `max_len` pulses, on a device holding
 `slots` wavefronts: segments tile the union exactly (K segments of a whole number of chunks, at
@@ -18,7 +18,7 @@ from conftest import ROOT
 
 DRIVER = r'''
 #include <cstdio>
-#include "sched.h"
+#include "serve_policy.h"
 int main() {
   long long live, slots; int max_len, chunk, max_seg;
   while (std::scanf("%lld %d %lld %d %d", &live, &max_len, &slots, &chunk, &max_seg) == 5) {
