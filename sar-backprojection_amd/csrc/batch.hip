@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "batch_internal.h"
+#include "serve_policy.h"
 
 namespace {
 constexpr size_t ALIGN = 256;
@@ -222,36 +223,7 @@ BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
 }
 
 void build_queues(const std::vector<WorkItemCost> &work, std::vector<BatchItem> &items, std::vector<int32_t> &seg) {
-  // Longest first (a counting sort over 32-pulse cost classes, list order within a class: the
-  // scheduling order only balances the load, results do not depend on it); then dealt to the
-  // queues in runs of K consecutive items, so each queue (XCD) works through neighbouring tiles and
-  // every queue sees the expensive items early.
-  constexpr int NB = 1024;
-  constexpr size_t K = 8;
-  auto cls = [](int64_t c) { return NB - 1 - (int)std::min<int64_t>(NB - 1, std::max<int64_t>(0, c) >> 5); };
-  const size_t n = work.size();
-  std::vector<int32_t> start(NB + 1, 0);
-  for (size_t k = 0; k < n; ++k) start[(size_t)cls(work[k].cost)]++;
-  int32_t acc = 0;
-  for (int c = 0; c <= NB; ++c) {
-    const int32_t m = c < NB ? start[(size_t)c] : 0;
-    start[(size_t)c] = acc;
-    acc += m;
-  }
-  std::vector<int32_t> order(n);
-  for (size_t k = 0; k < n; ++k) order[(size_t)start[(size_t)cls(work[k].cost)]++] = (int32_t)k;
-  size_t cnt[SARBP_BATCH_QUEUES] = {}, pos[SARBP_BATCH_QUEUES];
-  for (size_t k = 0; k < n; ++k) cnt[(k / K) % SARBP_BATCH_QUEUES]++;
-  seg.assign(SARBP_BATCH_QUEUES + 1, 0);
-  size_t o = 0;
-  for (int q = 0; q < SARBP_BATCH_QUEUES; ++q) {
-    seg[(size_t)q] = (int32_t)o;
-    pos[q] = o;
-    o += cnt[q];
-  }
-  seg[SARBP_BATCH_QUEUES] = (int32_t)o;
-  items.resize(n);
-  for (size_t k = 0; k < n; ++k) items[pos[(k / K) % SARBP_BATCH_QUEUES]++] = work[(size_t)order[k]].item;
+  deal_queues<SARBP_BATCH_QUEUES>(work, items, seg);
 }
 
 int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,

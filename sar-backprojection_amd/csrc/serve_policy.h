@@ -1,9 +1,11 @@
-// Scheduling rules of the serving loop that depend on integers only (no HIP): host code, also
-// compiled on its own by the CPU tests.
+// Scheduling rules of batched focusing and the serving loop that depend on integers only (no HIP):
+// host code, also compiled on its own by the CPU tests.
 #pragma once
 
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
+#include <vector>
 
 // Pulse segments per tile for a request of `live` tiles (union length > 0) on a device that holds
 // `slots` wavefronts of the request's kernel at once.  The resident wavefronts take the items
@@ -27,4 +29,40 @@ inline int segments_for(int64_t live, int max_len, int64_t slots, int chunk, int
   const int len = (int)((per + chunk - 1) / chunk * chunk);
   *seg_len = len;
   return (max_len + len - 1) / len;
+}
+
+// Work list of a persistent launch: items (with .item and .cost, the pulses to walk) in NQ queues.
+// Longest first (a counting sort over 32-pulse cost classes, list order within a class: the
+// scheduling order only balances the load, results do not depend on it); then dealt to the
+// queues in runs of 8 consecutive items, so each queue (XCD) works through neighbouring tiles and
+// every queue sees the expensive items early.  items: the queues one after the other; seg: NQ + 1
+// offsets into items.
+template <int NQ, class Work, class Item>
+void deal_queues(const std::vector<Work> &work, std::vector<Item> &items, std::vector<int32_t> &seg) {
+  constexpr int NB = 1024;
+  constexpr size_t K = 8;
+  auto cls = [](int64_t c) { return NB - 1 - (int)std::min<int64_t>(NB - 1, std::max<int64_t>(0, c) >> 5); };
+  const size_t n = work.size();
+  std::vector<int32_t> start(NB + 1, 0);
+  for (size_t k = 0; k < n; ++k) start[(size_t)cls(work[k].cost)]++;
+  int32_t acc = 0;
+  for (int c = 0; c <= NB; ++c) {
+    const int32_t m = c < NB ? start[(size_t)c] : 0;
+    start[(size_t)c] = acc;
+    acc += m;
+  }
+  std::vector<int32_t> order(n);
+  for (size_t k = 0; k < n; ++k) order[(size_t)start[(size_t)cls(work[k].cost)]++] = (int32_t)k;
+  size_t cnt[NQ] = {}, pos[NQ];
+  for (size_t k = 0; k < n; ++k) cnt[(k / K) % NQ]++;
+  seg.assign(NQ + 1, 0);
+  size_t o = 0;
+  for (int q = 0; q < NQ; ++q) {
+    seg[(size_t)q] = (int32_t)o;
+    pos[q] = o;
+    o += cnt[q];
+  }
+  seg[NQ] = (int32_t)o;
+  items.resize(n);
+  for (size_t k = 0; k < n; ++k) items[pos[(k / K) % NQ]++] = work[(size_t)order[k]].item;
 }
