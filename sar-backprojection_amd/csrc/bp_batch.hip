@@ -1,14 +1,15 @@
 // Persistent batch kernel: many requests (pixel sets of uneven size, aperture length and
-// precision) focused in one launch against one device-resident dataset.
+// precision, each with its own device-resident dataset) focused in one launch.
 //
-// The grid is sized to the device (blocks resident at once); each block repeatedly takes the next
-// work item from one of SARBP_BATCH_QUEUES queues -- its home queue (block % 8, the XCD the block
-// runs on) first, then the others -- until every queue is drained, so every block reaches the exit.
-// An item is W adjacent tiles of one request, one per wavefront; each wavefront focuses its tile
-// over the item's pulse range with the FP64 or split terms of bp_tile.h, starting from zero or from
-// the request's saved per-pixel state, and either writes the pixel means (last range of the tile)
-// or saves the state for a later launch.  The arithmetic per pixel is the same whatever the item
-// boundaries or the other requests in the launch, so a request's result does not depend on the mix.
+// The grid is the number of blocks the device holds at once, at most one per work item; each block
+// repeatedly takes the next item from one of SARBP_BATCH_QUEUES queues -- its home queue (block % 8,
+// the XCD the block runs on) first, then the others -- until every queue is drained, so every block
+// reaches the exit.  An item is W adjacent tiles of one request, one per wavefront; each wavefront
+// focuses its tile over the item's pulse range with the FP64 or split terms of bp_tile.h, starting
+// from zero or from the request's saved per-pixel state, and writes the pixel means (last range of
+// the tile), saves the state for a later launch, or -- for a pulse segment -- stores the segment's
+// partial sums for k_bp_combine.  The arithmetic per pixel is the same whatever the other requests
+// in the launch, so a request's result does not depend on the mix.
 #include <algorithm>
 #include <climits>
 
