@@ -95,3 +95,23 @@ def test_ctypes_struct_layouts_match_header(tmp_path):
         assert ctypes.sizeof(cls) == want[(pyname, "size")], pyname
         for f, _ in cls._fields_:
             assert getattr(cls, f).offset == want[(pyname, f)], (pyname, f)
+
+
+def test_product_path_fails_loudly_without_a_device():
+    """No CPU fallback: with no HIP device visible every focusing entry point raises, and argument
+    errors still raise ValueError first, as the reference does (sar_focuser_puntual.py:174-183)."""
+    if _lib.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    from sarbp import synth
+    from sarbp.core import Dataset
+    from sarbp.sar_focuser_puntual import FocalizzatoreBpPx
+    sc = synth.tiny_scene(seed=1)
+    with pytest.raises(_lib.SarbpError) as e:
+        Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    assert e.value.status == _lib.ENODEV
+    with pytest.raises(_lib.SarbpError):
+        FocalizzatoreBpPx(*sc.args())
+    args = list(sc.args())
+    args[0] = np.asarray(args[0]).ravel()  # xxg1 not 2-D
+    with pytest.raises(ValueError):
+        FocalizzatoreBpPx(*args)
