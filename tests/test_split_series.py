@@ -1,0 +1,77 @@
+"""The range series of the split form against exact distances (CPU, FP64).
+
+Per (pixel, pulse) the split kernels replace the slant-range difference |Pc + o - T| - |Pc - T| by
+    dR = a + t2 (1/(2R0) - a/(2R0^2) [- (|o|^2 - 5a^2)/(8R0^3)]),   a = u.o,  t2 = |o|^2 - a^2,
+(u the unit vector from the sensor T to the row's reference point Pc, R0 = |Pc - T|; the bracketed
+fourth-order term is used for offsets beyond SPLIT_FOURTH_M).  This checks that series, evaluated in
+FP64, against the exact difference over the geometry of the 4000 x 250 workload (slant ranges
+4.2-6 km, squint within +-25 degrees), with offsets in any direction up to the limits read from
+csrc/bp_tile.h: the truncation error, as phase (4 pi / lambda x error), stays below 2e-5 rad with
+the second-order series up to SPLIT_FOURTH_M (measured 1.4e-5) and below 4e-5 rad with the
+fourth-order term up to SPLIT_OMAX_M (measured 2.7e-5) -- at worst of the order of the split
+form's FP32 rounding (~3e-5 rad RMS), and orders of magnitude below it for the along-track row
+offsets of a regular grid.  A dropped or mis-signed term, or a wrong coefficient, exceeds the same
+bounds twenty-fold or more (checked below on planted variants).
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+K = 4 * np.pi / 0.031  # rad per metre of slant range
+
+
+def _limits():
+    text = open(os.path.join(ROOT, "sar-backprojection_amd", "csrc", "bp_tile.h")).read()
+    get = lambda name: float(re.search(rf"{name}\s*=\s*([0-9.]+)f", text).group(1))  # noqa: E731
+    return get("SPLIT_FOURTH_M"), get("SPLIT_OMAX_M")
+
+
+def _series(D, o, fourth, c_a=-0.5, c_4=5.0):
+    r0 = np.linalg.norm(D, axis=-1)
+    u = D / r0[:, None]
+    a = (u * o).sum(-1)
+    q = (o * o).sum(-1)
+    t2 = q - a * a
+    c = 1.0 / (2 * r0) + c_a * a / r0 ** 2
+    if fourth:
+        c = c - (q - c_4 * a * a) / (8 * r0 ** 3)
+    return a + t2 * c
+
+
+def _geometry(n, radius, seed):
+    rng = np.random.default_rng(seed)
+    px, py, pz = rng.uniform(0, 4000, n), rng.uniform(0, 1000, n), rng.uniform(0, 50, n)
+    sq = np.radians(rng.uniform(-25, 25, n))
+    rg = np.hypot(py + 3000, 3000 - pz)
+    T = np.stack([px - rg * np.tan(sq), np.full(n, -3000.0), np.full(n, 3000.0)], 1)
+    D = np.stack([px, py, pz], 1) - T
+    # offsets on the sphere of the given radius (the worst case at that distance) and inside it
+    o = rng.normal(size=(n, 3))
+    o *= (radius * rng.uniform(0.5, 1.0, n) ** (1 / 3) / np.linalg.norm(o, axis=1))[:, None]
+    o[: n // 2] *= (radius / np.linalg.norm(o[: n // 2], axis=1))[:, None]
+    exact = np.linalg.norm(D + o, axis=1) - np.linalg.norm(D, axis=1)
+    return D, o, exact
+
+
+@pytest.mark.parametrize("fourth", [False, True])
+def test_series_truncation_within_budget(fourth):
+    r4, rmax = _limits()
+    radius = rmax if fourth else r4
+    D, o, exact = _geometry(200_000, radius, seed=11 + fourth)
+    err = np.abs(_series(D, o, fourth) - exact) * K
+    print("fourth" if fourth else "second", "order up to", radius, "m: max", err.max(), "rad")
+    assert err.max() < (4e-5 if fourth else 2e-5)
+
+
+def test_planted_mistakes_fail_the_bound():
+    r4, rmax = _limits()
+    D, o, exact = _geometry(50_000, r4, seed=3)
+    for kw in ({"c_a": +0.5}, {"c_a": -0.45}):  # sign or coefficient of the a/(2R0^2) term
+        assert (np.abs(_series(D, o, False, **kw) - exact) * K).max() > 20 * 2e-5, kw
+    D, o, exact = _geometry(50_000, rmax, seed=4)
+    assert (np.abs(_series(D, o, False) - exact) * K).max() > 20 * 4e-5  # fourth-order term dropped at 48 m
+    assert (np.abs(_series(D, o, True, c_4=4.0) - exact) * K).max() > 20 * 4e-5  # wrong coefficient
