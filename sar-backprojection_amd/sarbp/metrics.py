@@ -14,11 +14,11 @@ scripts, computed on arrays (or on files in the output formats the focusers writ
   (||test| - |ref||_2 / ||ref||_2) and the largest per-pixel relative amplitude error.
 - :func:`target_peaks`: point-target responses of two images of one scene: same peak pixel,
   relative peak-height difference.
-
-Plotting, which the reference scripts add with matplotlib, is not part of these reports.
+- :func:`plot_compare_phase` and :func:`plot_coherence`: the figures of the two scripts (phases,
+  amplitudes, difference maps, histogram), written as PNG with matplotlib (Agg).
 
     python -m sarbp.metrics --file1 A_InterleavedF32.dat --file2 B.npy --shape 4000,250 \
-        --fmt1 interleaved --fmt2 npy [--no-mask] [--out prefix]
+        --fmt1 interleaved --fmt2 npy [--no-mask] [--out prefix [--plot]]
 """
 from __future__ import annotations
 
@@ -169,6 +169,84 @@ def target_peaks(test, ref, targets_xy, x0, y0, dx, dy, half_x=3, half_y=2, nois
             "peak_height_rel_err_max": float(rel.max()) if used else None}
 
 
+def _pyplot():
+    import matplotlib
+    matplotlib.use("Agg")
+    import matplotlib.pyplot as plt
+    return plt
+
+
+def plot_compare_phase(arr1, arr2, out_png, mask_zero_amp: bool = True):
+    """The 2 x 3 figure of compare_phase.py:123-173: phase of each image, the masked wrapped phase
+    difference map, the amplitude of each image and the histogram of the difference."""
+    plt = _pyplot()
+    arr1, arr2 = np.asarray(arr1), np.asarray(arr2)
+    d = phase_difference(arr1, arr2)
+    if mask_zero_amp:
+        d = np.where((np.abs(arr1) > ZERO_AMPLITUDE) & (np.abs(arr2) > ZERO_AMPLITUDE), d, np.nan)
+    fig = plt.figure(figsize=(18, 10))
+    panels = ((np.angle(arr1), "twilight", "Phase file 1", "Phase (rad)", {}),
+              (np.angle(arr2), "twilight", "Phase file 2", "Phase (rad)", {}),
+              (d, "twilight", "Phase difference map", "Phase diff (rad)", {"vmin": -np.pi, "vmax": np.pi}),
+              (np.abs(arr1), "gray", "Amplitude file 1", "Amplitude", {}),
+              (np.abs(arr2), "gray", "Amplitude file 2", "Amplitude", {}))
+    for k, (img, cmap, title, label, kw) in enumerate(panels):
+        plt.subplot(2, 3, k + 1)
+        plt.imshow(img, cmap=cmap, aspect="auto", **kw)
+        plt.colorbar(label=label)
+        plt.title(title)
+    plt.subplot(2, 3, 6)
+    v = d[np.isfinite(d)].ravel()
+    rng = None  # the data's extent, or a small window around a (numerically) constant difference
+    if v.size == 0 or float(v.max() - v.min()) <= 1e-12:
+        c = float(v.mean()) if v.size else 0.0
+        rng = (c - 1e-6, c + 1e-6)
+    plt.hist(v, bins=100, range=rng, color="C0", alpha=0.7)
+    plt.xlabel("Phase diff (rad)")
+    plt.ylabel("Count")
+    plt.title("Phase diff histogram")
+    plt.tight_layout()
+    fig.savefig(out_png, dpi=100)
+    plt.close(fig)
+    return out_png
+
+
+def plot_coherence(ref, test, out_png, threshold_fraction: float = PHASE_MAGNITUDE_THRESHOLD_FRACTION):
+    """The 2 x 3 figure of choerence_measure.py:231-301: magnitudes of the reference and the test
+    image (1st-99th percentile scale), |magnitude difference|, both phases and the |wrapped phase
+    difference| over the pixels above the magnitude threshold."""
+    plt = _pyplot()
+    ref, test = np.asarray(ref).astype(np.complex128), np.asarray(test)
+    mag_r, mag_t = np.abs(ref), np.abs(test)
+    dph = np.abs(wrap_phase(np.angle(test) - np.angle(ref)))
+    thr = threshold_fraction * (float(np.max(mag_r)) if np.any(mag_r) else 1.0)
+    valid = (mag_r > thr) & (mag_t > thr)
+    pos = mag_r[mag_r > 1e-9]
+    lo = float(np.percentile(pos if pos.size else mag_r, 1))
+    hi = float(np.percentile(mag_r, 99))
+    fig, ax = plt.subplots(2, 3, figsize=(18, 10), sharex=True, sharey=True)
+    ax[0, 0].imshow(mag_r, cmap="gray", vmin=lo, vmax=hi, aspect="auto")
+    ax[0, 0].set_title("Reference magnitude")
+    ax[0, 1].imshow(mag_t, cmap="gray", vmin=lo, vmax=hi, aspect="auto")
+    ax[0, 1].set_title("Test magnitude")
+    im = ax[0, 2].imshow(np.abs(mag_t - mag_r), cmap="magma", aspect="auto")
+    ax[0, 2].set_title("|Magnitude difference|")
+    fig.colorbar(im, ax=ax[0, 2], fraction=0.046, pad=0.04)
+    ax[1, 0].imshow(np.angle(ref), cmap="hsv", vmin=-np.pi, vmax=np.pi, aspect="auto")
+    ax[1, 0].set_title("Reference phase")
+    ax[1, 1].imshow(np.angle(test), cmap="hsv", vmin=-np.pi, vmax=np.pi, aspect="auto")
+    ax[1, 1].set_title("Test phase")
+    shown = np.where(valid, dph, np.nan)
+    vmax = float(np.percentile(dph[valid], 99.5)) if valid.any() else np.pi
+    im = ax[1, 2].imshow(shown, cmap="viridis", vmin=0, vmax=vmax or np.pi, aspect="auto")
+    ax[1, 2].set_title("|Wrapped phase difference| (valid pixels)")
+    fig.colorbar(im, ax=ax[1, 2], fraction=0.046, pad=0.04)
+    fig.tight_layout()
+    fig.savefig(out_png, dpi=100)
+    plt.close(fig)
+    return out_png
+
+
 def load_image(path: str, shape, fmt: str):
     """A complex image in one of the formats the focusers and the reference tools use:
     ``interleaved`` / ``split`` float32 (compare_phase.py:8-31), ``truth`` complex64
@@ -211,6 +289,7 @@ def _main(argv=None):
     ap.add_argument("--fmt2", default="truth", choices=fmts)
     ap.add_argument("--no-mask", action="store_true", help="do not mask zero amplitudes")
     ap.add_argument("--out", default=None, help="prefix of the <prefix>_statistics.json file")
+    ap.add_argument("--plot", action="store_true", help="also write <prefix>_phase_diff.png and <prefix>_comparison.png")
     a = ap.parse_args(argv)
     shape = tuple(map(int, a.shape.split(",")))
     img1, img2 = load_image(a.file1, shape, a.fmt1), load_image(a.file2, shape, a.fmt2)
@@ -218,6 +297,9 @@ def _main(argv=None):
     if a.out:
         with open(f"{a.out}_statistics.json", "w") as f:
             json.dump(rep["compare_phase"], f, indent=2)
+        if a.plot:
+            rep["figures"] = [plot_compare_phase(img1, img2, f"{a.out}_phase_diff.png", not a.no_mask),
+                              plot_coherence(img2, img1, f"{a.out}_comparison.png")]
     print(json.dumps(rep, indent=2))
     return 0 if rep["compare_phase"] is not None else 1
 

@@ -165,3 +165,20 @@ def test_load_image_formats_and_cli(tmp_path):
         saved = json.load(f)
     assert saved["valid_pixels"] == img.size and saved["rms_rad"] == 0.0
     assert os.path.exists(out + "_statistics.json")
+
+
+def test_figures_are_written(tmp_path):
+    pytest.importorskip("matplotlib")
+    ref = _img((30, 20), seed=6)
+    test = ref * np.exp(1j * 0.2)
+    for fn in (metrics.plot_compare_phase(test, ref, str(tmp_path / "p.png")),
+               metrics.plot_coherence(ref, test, str(tmp_path / "c.png"))):
+        with open(fn, "rb") as f:
+            head = f.read(8)
+        assert head == b"\x89PNG\r\n\x1a\n" and os.path.getsize(fn) > 10000
+    base = str(tmp_path / "Slc")
+    formats.write_outputs(ref, base)
+    out = str(tmp_path / "cmp")
+    assert metrics._main(["--file1", base + "_SplitF32.dat", "--file2", base + ".npy", "--shape", "30,20",
+                          "--fmt1", "split", "--fmt2", "npy", "--out", out, "--plot"]) == 0
+    assert os.path.exists(out + "_phase_diff.png") and os.path.exists(out + "_comparison.png")
