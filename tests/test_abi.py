@@ -115,3 +115,19 @@ def test_product_path_fails_loudly_without_a_device():
     args[0] = np.asarray(args[0]).ravel()  # xxg1 not 2-D
     with pytest.raises(ValueError):
         FocalizzatoreBpPx(*args)
+
+
+def test_integration_stub_matches_the_header():
+    """The ctypes stub INTEGRATION.md gives a reference maintainer declares the entry points it binds
+    with the header's parameter counts and loads against the built library (declarations only)."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = text.split("## 3.")[1].split("```python")[1].split("```")[0]
+    decl = block.split("def focus_gpu")[0].replace('ctypes.CDLL("libsarbp.so")', "ctypes.CDLL(LIB)")
+    ns = {"LIB": _lib.LIB_PATH}
+    exec(decl, ns)  # noqa: S102 -- the repository's own documentation
+    L = ns["_L"]
+    hdr = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "sarbp.h")).read(), flags=re.S)
+    for name in ("sarbp_dataset_create", "sarbp_focus", "sarbp_dataset_destroy"):
+        params = re.search(rf"\b{name}\s*\(([^)]*)\)", hdr).group(1)
+        assert len(getattr(L, name).argtypes) == len([p for p in params.split(",") if p.strip()]), name
+    assert "def focus_gpu" in block
