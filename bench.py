@@ -54,7 +54,6 @@ def cpu_baseline(sc, strip_cols, serial_cols, gpu_strip):
     x0 = sc.shape[0] // 2 - strip_cols // 2
     st = sc.sub(x0, x0 + strip_cols)
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = min(cores, 16)
     cnt = lambda s: O.count_pp(*[np.asarray(a).flatten(order="F") for a in (s.mask, s.index, s.sint)], s.Naz)  # noqa
     pp = cnt(st)
     t0 = time.perf_counter()
@@ -66,6 +65,8 @@ def cpu_baseline(sc, strip_cols, serial_cols, gpu_strip):
     O.focus(*s1.args(), nthreads=1)
     dt1 = time.perf_counter() - t1
     base = {"value": pp / dt, "unit": "pixel·pulses/s", "cores": int(cores), "kind": "port",
+            "host_cpus": {"nproc": _nproc(), "os_cpu_count": os.cpu_count(), "affinity": int(cores),
+                          "omp_num_threads": os.environ.get("OMP_NUM_THREADS")},
             "sample": f"{strip_cols}x{sc.shape[1]} px strip (x {x0}..{x0 + strip_cols - 1}) of the same "
                       f"workload, {pp} pp, {dt:.2f} s on {cores} threads, C restatement of process_point "
                       f"(OpenMP over pixels)",
@@ -79,6 +80,14 @@ def cpu_baseline(sc, strip_cols, serial_cols, gpu_strip):
     zeros_ok = bool(np.all(got.ravel()[~nz] == 0))
     return base, {"strip_f64_max_rel_err": float(rel.max()) if nz.any() else 0.0,
                   "strip_zero_pixels_exact": zeros_ok, "strip_pixels_checked": int(nz.sum())}
+
+
+def _nproc():
+    import subprocess
+    try:
+        return int(subprocess.run(["nproc"], capture_output=True, text=True, check=True).stdout.strip())
+    except Exception:
+        return None
 
 
 def small_scene_leg(cores):

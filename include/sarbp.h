@@ -71,6 +71,9 @@ int sarbp_dataset_create_streamed(sarbp_fill_fn fill, void *user, int64_t nra, i
                                   const double *tx, const double *ty, const double *tz,
                                   double near_range, double dra, double lambda_val,
                                   sarbp_dataset **out);
+/* Release the dataset's device memory.  SARBP_ESTATE (and the dataset stays valid) while a server
+ * holds requests against it that are not done or failed: preempted requests count, since they may
+ * be resumed; destroying the server fails and releases them. */
 int sarbp_dataset_destroy(sarbp_dataset *ds);
 /* Device bytes held by the dataset. */
 int sarbp_dataset_bytes(const sarbp_dataset *ds, int64_t *bytes);
@@ -91,6 +94,13 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
  * 1 = FP32 blocks of 64 pulses folded into FP64.
  */
 int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_x, int accum);
+/*
+ * Lowest range bin that contributes, for the next focusing passes of the scene: process_point's
+ * ind_line_ra_min (sar_focuser_puntual.py:27, 106).  A pulse counts only when its bin lies in
+ * [max(bin_min, 0), Nra-1): the reference keeps bins >= ind_line_ra_min and then skips negative ones
+ * in its accumulation loop (:138).  Default 0.
+ */
+int sarbp_scene_set_bin_min(sarbp_scene *sc, int32_t bin_min);
 int sarbp_scene_destroy(sarbp_scene *sc);
 
 /*
@@ -224,7 +234,9 @@ typedef struct sarbp_server_stats {
 int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
 /* Copy the request's inputs (into page-locked host memory the server reuses) and queue it.
  * out_c128 (2*n float64) and ncontrib (n int32, nullable) receive the result when the request
- * completes and must stay valid until then.  SARBP_EBUDGET if the request alone exceeds the
+ * completes: they, and the dataset, must stay valid until the request is done or failed (or the
+ * server is destroyed); sarbp_dataset_destroy refuses a dataset with such requests.  SARBP_EINVAL
+ * for a split request whose tile rows x pulse_chunk records exceed the batch kernel's LDS.  SARBP_EBUDGET if the request alone exceeds the
  * budget.  The request's latency (sarbp_request_info: done_s - submit_s) starts at the entry to
  * this call.  Once a request is done or failed the server keeps only its record (state, timings,
  * counts): its copy of the inputs is released. */

@@ -37,8 +37,10 @@ def aperture(index: int, sint: int) -> tuple[int, int]:
     return int(index) - h, J
 
 
-def process_point(x, y, z, mask, index, sint, tx, ty, tz, raw, near_range, dra, lambda_val):
-    """One pixel; ``raw`` is a complex64 array of shape (Nra, Naz).  Returns (complex, n)."""
+def process_point(x, y, z, mask, index, sint, tx, ty, tz, raw, near_range, dra, lambda_val, ind_line_ra_min=0):
+    """One pixel; ``raw`` is a complex64 array of shape (Nra, Naz).  Returns (complex, n).
+    ``ind_line_ra_min``: lowest range bin kept by the filter (sar_focuser_puntual.py:106); the
+    accumulation then skips bins outside [0, Nra) (:138)."""
     Nra, Naz = raw.shape
     if mask != 1 or sint == -9999:
         return 0j, 0
@@ -55,7 +57,7 @@ def process_point(x, y, z, mask, index, sint, tx, ty, tz, raw, near_range, dra, 
     r2 = dx * dx + dy * dy + dz * dz
     r = np.sqrt(r2)
     rb = np.rint((r - near_range) / dra)  # half-to-even, as np.round
-    keep = (rb < Nra - 1) & (rb >= 0)
+    keep = (rb < Nra - 1) & (rb >= ind_line_ra_min) & (rb >= 0)
     if not np.any(keep):
         return 0j, 0
     r = r[keep]

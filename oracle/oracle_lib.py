@@ -39,6 +39,9 @@ def lib():
         L.sarbp_oracle_point.argtypes = [_d, _d, _d, ctypes.c_uint8, _i32, _i32, _p, _p, _p, _p, _i64,
                                          _i64, _d, _d, _d, ctypes.POINTER(_d), ctypes.POINTER(_d)]
         L.sarbp_oracle_point.restype = _i32
+        L.sarbp_oracle_point_ex.argtypes = [_d, _d, _d, ctypes.c_uint8, _i32, _i32, _p, _p, _p, _p, _i64,
+                                            _i64, _d, _d, _d, _i32, ctypes.POINTER(_d), ctypes.POINTER(_d)]
+        L.sarbp_oracle_point_ex.restype = _i32
         L.sarbp_oracle_trace.argtypes = [_d, _d, _d, _i32, _i32, _p, _p, _p, _i64, _i64, _d, _d, _p,
                                          _p, _p, _i64]
         L.sarbp_oracle_trace.restype = _i64
@@ -78,6 +81,19 @@ def focus_flat(x, y, z, mask, index, sint, tx, ty, tz, raw, near, dra, lam, nthr
                              tz.ctypes.data, rawf.ctypes.data, Nra, Naz, float(near), float(dra),
                              float(lam), out.ctypes.data, cnt.ctypes.data, int(nthreads))
     return out, cnt
+
+
+def point(x, y, z, mask, index, sint, tx, ty, tz, raw, near, dra, lam, ind_line_ra_min=0):
+    """process_point (sar_focuser_puntual.py:21-155) for one pixel, with its ind_line_ra_min
+    argument: (complex128, contributor count)."""
+    tx, ty, tz = _c(tx, np.float64), _c(ty, np.float64), _c(tz, np.float64)
+    rawf = _raw_f32(raw)
+    Nra, Naz = rawf.shape
+    re, im = _d(), _d()
+    n = lib().sarbp_oracle_point_ex(float(x), float(y), float(z), int(mask), int(index), int(sint), tx.ctypes.data,
+                                    ty.ctypes.data, tz.ctypes.data, rawf.ctypes.data, Nra, Naz, float(near),
+                                    float(dra), float(lam), int(ind_line_ra_min), ctypes.byref(re), ctypes.byref(im))
+    return complex(re.value, im.value), int(n)
 
 
 def focus(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz, raw, near, dra, lam, Sint, nthreads=0):

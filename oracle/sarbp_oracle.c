@@ -47,13 +47,15 @@ static void orc_aperture(int32_t index, int32_t sint, int64_t *start, int64_t *c
     *count = J;
 }
 
-/* process_point, sar_focuser_puntual.py:21-155.  raw is interleaved complex64, column-major
- * (Nra, Naz): sample (r, a) at raw[2*(r + Nra*a)].  Returns the contributor count n. */
-int32_t sarbp_oracle_point(double x, double y, double z, uint8_t mask, int32_t index, int32_t sint,
-                           const double *tx, const double *ty, const double *tz,
-                           const float *raw, int64_t Nra, int64_t Naz,
-                           double near_range, double dra, double lambda_val,
-                           double *out_re, double *out_im)
+/* process_point, sar_focuser_puntual.py:21-155, with its ind_line_ra_min argument (sfp:27): the
+ * range-bin filter keeps bins >= ind_line_ra_min (sfp:106) and the accumulation loop then skips bins
+ * outside [0, Nra) (sfp:138).  raw is interleaved complex64, column-major (Nra, Naz): sample (r, a)
+ * at raw[2*(r + Nra*a)].  Returns the contributor count n. */
+int32_t sarbp_oracle_point_ex(double x, double y, double z, uint8_t mask, int32_t index, int32_t sint,
+                              const double *tx, const double *ty, const double *tz,
+                              const float *raw, int64_t Nra, int64_t Naz,
+                              double near_range, double dra, double lambda_val, int32_t ind_line_ra_min,
+                              double *out_re, double *out_im)
 {
     const double beta = 4.0 * ORC_PI * ORC_ALFA / (ORC_C * ORC_C); /* sfp:40 */
     const double kphase = -4.0 * ORC_PI / lambda_val;               /* sfp:116, left operand */
@@ -74,8 +76,9 @@ int32_t sarbp_oracle_point(double x, double y, double z, uint8_t mask, int32_t i
         double r2 = dx * dx + dy * dy + dz * dz;               /* sfp:88, left to right */
         double r = sqrt(r2);                                   /* sfp:94 */
         double q = nearbyint((r - near_range) / dra);          /* sfp:100-101, half-even */
-        if (!(q < (double)(Nra - 1) && q >= 0.0)) continue;    /* sfp:106 */
+        if (!(q < (double)(Nra - 1) && q >= (double)ind_line_ra_min)) continue; /* sfp:106 */
         int64_t rb = (int64_t)q;
+        if (!(rb >= 0 && rb < Nra)) continue;                  /* sfp:138 */
         double phase = kphase * r + beta * r2;                 /* sfp:116 */
         double fr = cos(phase);                                /* sfp:121: exp(-j*phase) */
         double fi = -sin(phase);
@@ -91,6 +94,17 @@ int32_t sarbp_oracle_point(double x, double y, double z, uint8_t mask, int32_t i
         *out_im = acc_im / (double)n;
     }
     return n;
+}
+
+/* process_point with ind_line_ra_min = 0, as FocalizzatoreBpPx calls it (sar_focuser_puntual.py:214). */
+int32_t sarbp_oracle_point(double x, double y, double z, uint8_t mask, int32_t index, int32_t sint,
+                           const double *tx, const double *ty, const double *tz,
+                           const float *raw, int64_t Nra, int64_t Naz,
+                           double near_range, double dra, double lambda_val,
+                           double *out_re, double *out_im)
+{
+    return sarbp_oracle_point_ex(x, y, z, mask, index, sint, tx, ty, tz, raw, Nra, Naz, near_range, dra,
+                                 lambda_val, 0, out_re, out_im);
 }
 
 /* The per-(pixel, pulse) arithmetic of process_point (sar_focuser_puntual.py:82-121) for n pairs:

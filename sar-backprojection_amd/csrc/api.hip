@@ -1,5 +1,6 @@
 // Host side of the C ABI declared in include/sarbp.h: device-resident datasets and pixel sets,
 // focusing passes, error reporting.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -52,6 +53,11 @@ int upload_traj(sarbp_dataset *ds, const double *tx, const double *ty, const dou
   ds->htx.assign(tx, tx + ds->naz);
   ds->hty.assign(ty, ty + ds->naz);
   ds->htz.assign(tz, tz + ds->naz);
+  const double *c[3] = {tx, ty, tz};
+  for (int k = 0; k < 3; ++k) {
+    ds->tlo[k] = *std::min_element(c[k], c[k] + ds->naz);
+    ds->thi[k] = *std::max_element(c[k], c[k] + ds->naz);
+  }
   std::vector<double4> t((size_t)ds->naz);
   for (int64_t a = 0; a < ds->naz; ++a) t[(size_t)a] = make_double4(tx[a], ty[a], tz[a], 0.0);
   return h2d(ds->traj, t.data(), t.size() * sizeof(double4), ds->stream, "trajectory upload");
@@ -129,6 +135,26 @@ Tiling default_tiling(int precision) {
   return t;
 }
 
+float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n) {
+  if (n <= 0) return 0.f;
+  const double *c[3] = {x, y, z};
+  double d2 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double lo = *std::min_element(c[k], c[k] + n), hi = *std::max_element(c[k], c[k] + n);
+    const double gap = std::max({0.0, lo - ds->thi[k], ds->tlo[k] - hi});
+    d2 += gap * gap;
+  }
+  const double d = std::sqrt(d2);
+  if (!std::isfinite(d)) return 0.f;
+  return std::nextafter((float)d, 0.f);  // rounded down
+}
+
+BpGeom sarbp_scene::geom() const {
+  BpGeom g = ds->geom();
+  g.bin_lo = bin_lo;
+  return g;
+}
+
 BpGeom sarbp_dataset::geom() const {
   BpGeom g;
   g.raw = raw;
@@ -143,6 +169,7 @@ BpGeom sarbp_dataset::geom() const {
   g.kphase = -4.0 * pi / lambda_val;
   g.beta = 4.0 * pi * SARBP_ALFA_F64 / (SARBP_C_F64 * SARBP_C_F64);
   g.sctab = sctab;
+  g.bin_lo = 0;
   return g;
 }
 
@@ -237,6 +264,8 @@ int sarbp_dataset_create_streamed(sarbp_fill_fn fill, void *user, int64_t nra, i
 
 int sarbp_dataset_destroy(sarbp_dataset *ds) {
   if (!ds) return SARBP_OK;
+  if (ds->server_refs.load() > 0)
+    return sarbp_fail(SARBP_ESTATE, "the dataset has server requests that are not done or failed");
   if (ds->stream) (void)hipStreamSynchronize(ds->stream);
   dev_free(ds->raw);
   dev_free(ds->traj);
@@ -336,6 +365,7 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
     sarbp_scene_destroy(sc);
     return rc;
   }
+  sc->r0min = sarbp_pixel_r0min(ds, x, y, z, n);
   *out = sc;
   return SARBP_OK;
 }
@@ -405,6 +435,15 @@ int sarbp_scene_set_tiling(sarbp_scene *sc, int precision, int kernel, int tile_
   t.kernel = kernel;
   t.tile_x = tile_x;
   t.accum = accum;
+  return SARBP_OK;
+}
+
+int sarbp_scene_set_bin_min(sarbp_scene *sc, int32_t bin_min) {
+  if (!sc) return sarbp_fail(SARBP_EINVAL, "null handle");
+  // bins below 0 never contribute (process_point's explicit bound check, sar_focuser_puntual.py:138),
+  // and none at or above nra-1 (:106)
+  const int64_t hi = sc->ds->nra - 1;
+  sc->bin_lo = (int32_t)std::min<int64_t>(hi, std::max<int64_t>(0, bin_min));
   return SARBP_OK;
 }
 

@@ -17,7 +17,8 @@ constexpr size_t ALIGN = 256;
 size_t up(size_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
 std::mutex g_tune_mu;
-std::map<std::tuple<int64_t, int64_t, int64_t, int32_t>, std::pair<int, int>> g_tuned;  // -> (tile_px, chunk)
+// (nra, naz, pixels, requests, rows of the widest split tile) -> (tile_px, chunk)
+std::map<std::tuple<int64_t, int64_t, int64_t, int32_t, int>, std::pair<int, int>> g_tuned;
 }  // namespace
 
 char *DevicePool::get(size_t bytes, size_t *cap) {
@@ -72,6 +73,7 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
   pp = 0;
   if (rows) *rows = SplitRows();
   if (r.n <= 0) return;
+  if (rows) rows->r0min = sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n);
   const double *htx = ds->htx.data(), *hty = ds->hty.data(), *htz = ds->htz.data();
   for (int64_t by = 0; by < tg.nty; ++by) {
     const int64_t y0 = by * tg.ty, y1 = std::min(dimy, y0 + tg.ty);
@@ -119,7 +121,7 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
 }
 
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
-                  hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len) {
+                  hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len, float r0min) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
   if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
@@ -170,6 +172,8 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
   b.out = (double2 *)(p + off[6]);
   b.ncnt = (int32_t *)(p + off[7]);
   b.state = state_slots > 0 ? (BpState *)(p + off[8]) : nullptr;
+  b.r0min = r0min >= 0.f ? r0min
+                         : r.precision == SARBP_PREC_SPLIT ? sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n) : 0.f;
   const void *src[6] = {r.x, r.y, r.z, r.mask, r.index, r.sint};
   void *dst[6] = {b.x, b.y, b.z, b.mask, b.index, b.sint};
   for (int k = 0; k < 6 && n; ++k) {
@@ -212,7 +216,7 @@ void reqbuf_free(ReqBuf &b) {
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
   BatchReqDev d;
   d.g = ds->geom();
-  d.px = PixelsView{b.x, b.y, b.z, b.mask, b.index, b.sint, b.n};
+  d.px = PixelsView{b.x, b.y, b.z, b.mask, b.index, b.sint, b.n, b.r0min};
   d.out = b.out;
   d.ncnt = b.ncnt;
   d.state = b.state;
@@ -341,7 +345,7 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
 bool valid_combo(int tile_px, int chunk, int max_rows) {
   if (tile_px != 64 && tile_px != 128 && tile_px != 256) return false;
   if (chunk < 8 || chunk > 1024 || chunk % 8) return false;
-  return sarbp_batch_lds_bytes(tile_px / 64, chunk, max_rows) <= 64 * 1024;
+  return sarbp_batch_lds_bytes(tile_px / 64, chunk, max_rows) <= SARBP_BATCH_LDS_MAX;
 }
 }  // namespace
 
@@ -375,8 +379,8 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
   }
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    auto it = g_tuned.find(std::make_tuple(ds->nra, ds->naz, b->total_px, nreq));
-    if (it != g_tuned.end()) {
+    auto it = g_tuned.find(std::make_tuple(ds->nra, ds->naz, b->total_px, nreq, b->max_rows));
+    if (it != g_tuned.end() && valid_combo(it->second.first, it->second.second, b->max_rows)) {
       b->tile_px = it->second.first;
       b->chunk = it->second.second;
     }
@@ -418,7 +422,7 @@ int sarbp_batch_autotune(sarbp_batch *b, int32_t reps, int32_t *best_tile_px, in
   b->chunk = bc;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tuned[std::make_tuple(b->ds->nra, b->ds->naz, b->total_px, (int32_t)b->reqs.size())] = {bt, bc};
+    g_tuned[std::make_tuple(b->ds->nra, b->ds->naz, b->total_px, (int32_t)b->reqs.size(), b->max_rows)] = {bt, bc};
   }
   if (best_tile_px) *best_tile_px = bt;
   if (best_chunk) *best_chunk = bc;

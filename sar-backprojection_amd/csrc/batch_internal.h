@@ -35,6 +35,7 @@ struct ReqBuf {
   int32_t *ncnt = nullptr;
   BpState *state = nullptr;
   int64_t pp = 0;                 // pixel-pulse pairs of the request (0 when tile_len was given)
+  float r0min = 0.f;              // lower bound of the pixel-to-sensor distances (sarbp_pixel_r0min)
   std::vector<int> tile_len;      // union length of each tile (pulses)
 };
 
@@ -46,6 +47,7 @@ struct SplitRows {
   bool any = false;   // some row has a pixel with an aperture
   double amax = 0.0;  // largest o_x * c + o_y + o_z over the rows (metres)
   double omax = 0.0;  // largest |o| over the rows (metres)
+  double r0min = 0.0; // lower bound of the request's pixel-to-sensor distances (metres)
 };
 // Tile grid, per-tile union lengths (pulses) and pixel-pulse pairs of a request, in one pass over
 // the pixels tile by tile (host); with `rows`, also the tile rows' geometry above.
@@ -55,8 +57,10 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
 // BpState entries per pixel (0: none; serving: 1 for the state between rounds, or one per pulse
 // segment).  pool: where the device block comes from (null: hipMalloc); tile_len: precomputed by
 // request_tiles (null: computed here).
+// r0min: the pixels' distance bound when the caller has it (negative: computed here for split
+// requests; FP64 requests do not use it).
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b, hipStream_t st,
-                  DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr);
+                  DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr, float r0min = -1.f);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 

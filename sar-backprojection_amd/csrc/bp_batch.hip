@@ -50,7 +50,7 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
   bool split = MODE != SARBP_BATCH_F64 && r.precision == SARBP_PREC_SPLIT;
   SplitTile s;
   if (split) {
-    s = split_tile(g, r.tg, t);
+    s = split_tile(g, r.tg, t, r.px.r0min);
     split = s.use;  // not a compact tile: FP64 terms (same decision every range)
   }
   if (!split) {
@@ -201,7 +201,7 @@ int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d
   if (chunk < 8 || chunk > 1024 || chunk % 8) return sarbp_fail(SARBP_EINVAL, "pulse chunk must be 8..1024, x8");
   if (max_rows < 1 || max_rows > 64) return sarbp_fail(SARBP_EINVAL, "bad tile row count");
   const size_t lds = sarbp_batch_lds_bytes(waves, chunk, max_rows);
-  if (lds > 64 * 1024) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk x tile rows needs more than 64 KiB of LDS");
+  if (lds > SARBP_BATCH_LDS_MAX) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk x tile rows needs more than 64 KiB of LDS");
   if (nitems <= 0) return SARBP_OK;
   hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_HEADS_BYTES, st);
   if (e != hipSuccess) return sarbp_hip_fail(e, "queue reset");

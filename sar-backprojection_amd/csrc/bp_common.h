@@ -72,7 +72,7 @@ __device__ __forceinline__ bool f64_bin(const BpGeom &g, double x, double y, dou
   r2 = dx * dx + dy * dy + dz * dz;          // :88
   r = sqrt_rn(r2);                           // :94
   q = range_bin(g, r);                       // :100-101, half to even
-  return q < (double)(g.nra - 1) && q >= 0.0;  // :106
+  return q < (double)(g.nra - 1) && q >= (double)g.bin_lo;  // :106 (bin_lo: ind_line_ra_min, >= 0)
 }
 
 // sin and cos of the phase of sar_focuser_puntual.py:116: exp(-j ph) = (cs, -sn) (:121); tab is
@@ -126,6 +126,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_column(const BpGeom &g, in
   return raw_column_at(g.raw + (int64_t)a * g.nra, g.nra);
 }
 
+// Buffer resource of the split path's gathers: col points at range bin bin_lo of a pulse column, and
+// nbin_bytes = (nra - 1 - bin_lo) * 8 covers the contributing bins [bin_lo, nra-1).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_split_column(const float2 *col, unsigned nbin_bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)col, (short)0, (int)nbin_bytes, 0x00020000);
+}
+
 __device__ __forceinline__ float2 raw_load(__amdgpu_buffer_rsrc_t col, unsigned byte_off) {
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(col, (int)byte_off, 0, 0));
 }
@@ -141,7 +147,7 @@ __device__ __forceinline__ void store_mean(double2 *out, int32_t *ncnt, int64_t 
 }
 
 // Range bin of one pixel-pulse pair exactly as process_point selects it
-// (sar_focuser_puntual.py:82-106); -1 when outside [0, nra-1).
+// (sar_focuser_puntual.py:82-106); -1 when outside [bin_lo, nra-1).
 [[maybe_unused]] __device__ __forceinline__ int exact_bin(const BpGeom &g, double x, double y, double z, double tx, double ty,
                                       double tz) {
 #pragma clang fp contract(off)
@@ -151,7 +157,7 @@ __device__ __forceinline__ void store_mean(double2 *out, int32_t *ncnt, int64_t 
   const double r2 = dx * dx + dy * dy + dz * dz;
   const double r = __builtin_sqrt(r2);
   const double q = __builtin_rint((r - g.near_range) / g.dra);
-  return (q < (double)(g.nra - 1) && q >= 0.0) ? (int)q : -1;
+  return (q < (double)(g.nra - 1) && q >= (double)g.bin_lo) ? (int)q : -1;
 }
 
 }  // namespace

@@ -103,13 +103,13 @@ int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tilin
   if (t.kernel == 0) {
     const int64_t blocks = (sc->n + 255) / 256;
     if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many pixels for one launch");
-    hipLaunchKernelGGL(k_bp_f64_lane, dim3((unsigned)blocks), dim3(256), 0, st, ds->geom(), sc->view(), sc->out,
+    hipLaunchKernelGGL(k_bp_f64_lane, dim3((unsigned)blocks), dim3(256), 0, st, sc->geom(), sc->view(), sc->out,
                        sc->ncnt);
   } else {
     const TileGrid tg = make_tile_grid(sc->dimx, sc->dimy, t.tile_x);
     const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
     if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
-    hipLaunchKernelGGL(k_bp_f64_tile, dim3((unsigned)blocks), dim3(64), 0, st, ds->geom(), sc->view(), tg, sc->out,
+    hipLaunchKernelGGL(k_bp_f64_tile, dim3((unsigned)blocks), dim3(64), 0, st, sc->geom(), sc->view(), tg, sc->out,
                        sc->ncnt);
   }
   hipError_t e = hipGetLastError();

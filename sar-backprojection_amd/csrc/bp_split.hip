@@ -20,7 +20,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void
   double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
   if (t.wlo < t.whi) {
-    const SplitTile s = split_tile(g, tg, t);
+    const SplitTile s = split_tile(g, tg, t, px.r0min);
     if (!s.use) {  // not a compact tile: FP64 terms
       f64_span(g, g.sctab, t, t.wlo, t.whi, ar, ai, cnt);  // table read from global memory (rare path)
     } else {
@@ -40,10 +40,10 @@ int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Til
   const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
   if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
   if (t.accum == 0)
-    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, ds->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   else
-    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, ds->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "split kernel launch");

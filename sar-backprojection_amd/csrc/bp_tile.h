@@ -30,8 +30,6 @@
 
 namespace {
 
-constexpr float SPLIT_OMAX_M = 48.0f;    // largest pixel-to-centre offset focused in split form
-constexpr float SPLIT_FOURTH_M = 12.0f;  // include the 4th-order range term beyond this offset
 constexpr double INV_2PI = 0.15915494309189533577;
 
 // Tile of one pixel grid, as seen by one lane.
@@ -101,7 +99,7 @@ struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads
 extern __shared__ PulseRec sarbp_lds_rec[];
 
 __device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, double zc, double4 t,
-                                        PulseRec &r) {
+                                        PulseRec &r) {  // returns B0 (absolute bin)
   const double dx = xc - t.x, dy = yc - t.y, dz = zc - t.z;
   const double r0sq = dx * dx + dy * dy + dz * dz;
   // 1/R0 from the hardware reciprocal square root and two Newton steps (a few ulp; R0 is only the
@@ -120,7 +118,8 @@ __device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, d
   const double t0 = turns - __builtin_floor(turns);
   const double c1 = (g.kphase + 2.0 * g.beta * r0) * INV_2PI;
   r.a = make_float4((float)(dx * inv), (float)(dy * inv), (float)(dz * inv), (float)(0.5 * inv));
-  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float((int)b0 * 8), (float)t0);
+  // byte offset of bin B0 from bin bin_lo, where the split path's column resources start
+  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float(((int)b0 - g.bin_lo) * 8), (float)t0);
   r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
   return (int)b0;
 }
@@ -150,7 +149,8 @@ __device__ __forceinline__ double row_max_f64(double v, int tx) {
   return v;
 }
 
-__device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid &tg, const TileLane &t) {
+// r0min: lower bound of the tile's pixel-to-sensor distances (PixelsView::r0min).
+__device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid &tg, const TileLane &t, float r0min) {
   SplitTile s;
   const int lane = threadIdx.x & 63;
   const bool act = t.hi > t.lo;
@@ -173,10 +173,14 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   s.oz = act ? (float)(t.z - zc) : 0.f;
   s.qo = s.ox * s.ox + s.oy * s.oy + s.oz * s.oz;
   s.omax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(sqrtf((float)wave_max_f64((double)s.qo)))));
-  // error of f0 + dR/dra in bins: f0 and the FP32 sum ~1e-7, dR ~1.2e-7 * |o| m; margin >= 3x
-  s.guard = 0.5f - (2e-5f + 4e-7f * s.omax * (float)g.inv_dra);
-  // compact tiles only; and |dR|/dra well inside the int32 byte offsets of split_first
-  s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 16777216.0;
+  // error of f0 + dR/dra in bins: f0 and the FP32 sum ~1e-7, dR ~1.2e-7 * |o| m; margin >= 3x; plus
+  // the series truncation
+  const double trunc = split_truncation_m((double)s.omax, (double)r0min);
+  s.guard = 0.5f - (2e-5f + 4e-7f * s.omax * (float)g.inv_dra) - (float)(fmin(trunc, 1.0) * g.inv_dra);
+  // compact tiles only, whose series truncation stays far below the FP32 rounding; and |dR|/dra well
+  // inside the int32 byte offsets of split_first
+  s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 16777216.0 &&
+          trunc * __builtin_fabs(g.kphase) <= SPLIT_TRUNC_MAX_RAD;
   s.poison = act ? 0u : 0x80000000u;
   s.core_lo = __builtin_amdgcn_readfirstlane(wave_max_i32(act ? t.lo : INT_MIN));
   s.core_hi = __builtin_amdgcn_readfirstlane(wave_min_i32(act ? t.hi : INT_MAX));
@@ -256,13 +260,14 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
   unsigned off = ((unsigned)(int)rq << 3) + (unsigned)__float_as_int(rb.z);
   if (act && fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly (rare)
     const double4 tr = g.traj[a];
-    off = (unsigned)exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z) << 3;  // -1 -> out of range
+    const int eb = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);  // -1: outside [bin_lo, nra-1)
+    off = (unsigned)(eb - g.bin_lo) << 3;  // -1 - bin_lo -> far beyond the column
   }
   if (FAST) {
     h.bad = false;
     off |= s.poison;  // beyond the column for a lane without an aperture
   } else {
-    const bool ok = off < nbin_bytes;  // 0 <= bin < Nra-1
+    const bool ok = off < nbin_bytes;  // bin_lo <= bin < Nra-1
     h.bad = act && !ok;
     off = act && ok ? off : SPLIT_NO_SAMPLE;
   }
@@ -279,9 +284,10 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
                                             const PulseRec *rk, int rstep, int a, const float2 *col,
                                             float inv_dra, unsigned nbin_bytes, float beta_t, float &sr, float &si,
                                             float &cr, float &ci, int32_t &cnt) {
-  const int64_t nra = g.nra;  // col: the column of pulse a
-  const __amdgpu_buffer_rsrc_t c0r = raw_column_at(col, nra), c1r = raw_column_at(col + nra, nra),
-                               c2r = raw_column_at(col + 2 * nra, nra), c3r = raw_column_at(col + 3 * nra, nra);
+  const int64_t nra = g.nra;  // col: bin bin_lo of the column of pulse a
+  const __amdgpu_buffer_rsrc_t c0r = raw_split_column(col, nbin_bytes), c1r = raw_split_column(col + nra, nbin_bytes),
+                               c2r = raw_split_column(col + 2 * nra, nbin_bytes),
+                               c3r = raw_split_column(col + 3 * nra, nbin_bytes);
   SplitHalf X, Y, Z, W;
   split_first<FOURTH, FAST>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
   split_first<FOURTH, FAST>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
@@ -315,10 +321,10 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
   const int lane = threadIdx.x & 63;
   const float inv_dra = (float)g.inv_dra;
   const float beta_t = (float)(g.beta * INV_2PI);
-  const unsigned nbin_bytes = (unsigned)(g.nra - 1) * 8u;
+  const unsigned nbin_bytes = (unsigned)(g.nra - 1 - g.bin_lo) * 8u;
   const int nrec = C << s.ty_shift;
   const int rstep = 1 << s.ty_shift;
-  const int blo = s.margin, bhi = (int)(g.nra - 1) - 1 - s.margin;  // reference bins inside the swath
+  const int blo = g.bin_lo + s.margin, bhi = (int)(g.nra - 1) - 1 - s.margin;  // reference bins inside the swath
   // contributors: the pulses of this lane's aperture within [c0, p1), less the rare ones whose bin
   // falls outside the swath (counted as they occur)
   {
@@ -340,7 +346,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row;
-    const float2 *col = g.raw + (int64_t)c0 * g.nra;  // column of pulse c0 + k
+    const float2 *col = g.raw + (int64_t)c0 * g.nra + g.bin_lo;  // bin bin_lo of the column of pulse c0 + k
     int k = 0;
     for (; k + 3 < cn; k += 4, rk += 4 * rstep, col += 4 * g.nra) {
       const int a = c0 + k;
@@ -353,7 +359,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     }
     for (; k < cn; ++k, rk += rstep, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
-      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_column_at(col, g.nra), inv_dra, nbin_bytes, X);
+      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
       split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt -= X.bad ? 1 : 0;
     }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -28,6 +29,8 @@ struct BpGeom {
   double kphase;        // (-4*PI32)/lambda        (sar_focuser_puntual.py:116)
   double beta;          // ((4*PI32)*ALFA)/(C*C)   (sar_focuser_puntual.py:40)
   const double2 *sctab; // SARBP_SCTAB entries (sin, cos)(j*pi/128), j = 0..255, for sincos_tab
+  int32_t bin_lo;       // lowest range bin that contributes (process_point's ind_line_ra_min, clamped
+                        // to [0, nra-1]; 0 unless a scene sets it)
 };
 
 // Entries of the sin/cos table of the FP64 path (one turn in steps of pi/128).
@@ -42,6 +45,7 @@ struct PixelsView {
   const int32_t *index;
   const int32_t *sint;
   int64_t n;
+  float r0min;  // lower bound of every pixel-to-sensor distance (m), for the split form's series bound
 };
 
 struct sarbp_dataset {
@@ -55,7 +59,11 @@ struct sarbp_dataset {
   double4 *traj = nullptr;
   double2 *sctab = nullptr;           // SARBP_SCTAB (sin, cos) pairs
   std::vector<double> htx, hty, htz;  // host copy of the trajectory (request planning)
+  double tlo[3] = {0, 0, 0}, thi[3] = {0, 0, 0};  // bounding box of the trajectory
   int64_t bytes = 0;
+  // requests submitted to a server against this dataset and not yet done or failed (preempted ones
+  // included: they may be resumed); sarbp_dataset_destroy refuses while any is left
+  std::atomic<int64_t> server_refs{0};
   BpGeom geom() const;
 };
 
@@ -80,7 +88,10 @@ struct sarbp_scene {
   double2 *out = nullptr;
   int32_t *ncnt = nullptr;
   int64_t bytes = 0;
-  PixelsView view() const { return PixelsView{x, y, z, mask, index, sint, n}; }
+  int32_t bin_lo = 0;  // lowest contributing range bin (sarbp_scene_set_bin_min)
+  float r0min = 0.f;   // lower bound of the pixel-to-sensor distances (sarbp_pixel_r0min)
+  PixelsView view() const { return PixelsView{x, y, z, mask, index, sint, n, r0min}; }
+  BpGeom geom() const;  // the dataset's geometry with this scene's range-bin floor
 };
 
 // Aperture of one pixel (sar_focuser_puntual.py:57-66): J = 2*ceil(s/2) + 1 pulses starting at
@@ -104,6 +115,27 @@ __host__ __device__ inline void sarbp_pulse_range(int32_t index, int32_t sint, i
 }
 
 Tiling default_tiling(int precision);
+
+// Split form limits (bp_tile.h): the largest pixel-to-row-reference offset it focuses, and the offset
+// beyond which its range series includes the fourth-order term.
+constexpr float SPLIT_OMAX_M = 48.0f;
+constexpr float SPLIT_FOURTH_M = 12.0f;
+// Bound of the range series' truncation (m) for offsets up to omax from the reference point, at
+// reference ranges of at least r0min: omax^4/(6.4 r0min^3) for the third-order form,
+// omax^5/(8 r0min^4) with the fourth-order term (the suprema over offset directions are
+// omax^4/(8 R0^3) and 0.083 omax^5/R0^4 while omax <= R0/6; checked in FP64 against exact distances
+// on the CPU).  Host and device.
+__host__ __device__ inline double split_truncation_m(double omax, double r0min) {
+  if (!(r0min > 6.0 * omax)) return 1e30;  // the series needs |o| well inside R0
+  const double q = omax / r0min, o4 = q * q * q * q * r0min;
+  return omax > (double)SPLIT_FOURTH_M ? o4 * q / 8.0 : o4 / 6.4;  // margins 1.5x and 1.25x
+}
+// Phase (rad) that the truncation may reach before a tile takes the FP64 terms instead.
+constexpr double SPLIT_TRUNC_MAX_RAD = 1e-4;
+
+// Lower bound of the distance from any of n pixels to any trajectory sample of ds: the distance
+// between the two bounding boxes (0 when they overlap), rounded down to float.
+float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n);
 
 
 
@@ -231,6 +263,7 @@ int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d
                        int32_t *d_heads, int nitems, int mode, int waves, int chunk, int max_rows, int grid_blocks,
                        hipStream_t st);
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows);
+constexpr size_t SARBP_BATCH_LDS_MAX = 64 * 1024;  // LDS of one batch block's pulse records
 // Blocks of `waves` wavefronts of the batch kernel for `mode` that the device holds at once.
 int sarbp_batch_resident_blocks(int device, int mode, int waves, int chunk, int max_rows, int *blocks);
 // Pixel means from the partial sums of nseg pulse segments (part[k * n + i], added in the order

@@ -90,14 +90,20 @@ struct ServerReq {
   int32_t state = SARBP_REQ_QUEUED;
   int32_t precision = SARBP_PREC_F64;
   double est = 0.0;
+  float r0min = 0.f;  // the pixels' distance bound (split requests; from the submission pass)
   ReqBuf buf;
   bool on_device = false;
   int off = 0;       // pulses of every tile's union already focused (multiple of the chunk)
   int max_len = 0;   // longest tile union (pulses)
   int64_t need = 0;  // device bytes while admitted
-  std::vector<BpState> saved;  // partial state of a preempted request
+  // a preempted request's device contents, saved on the host: every pixel's partial state, and the
+  // means and counts already written by the tiles that finished before the preemption
+  std::vector<BpState> saved;
+  std::vector<double2> saved_out;
+  std::vector<int32_t> saved_cnt;
   std::vector<int> tile_len;   // per-tile union lengths, computed at submission (caller's thread)
   bool preempt = false;
+  bool holds_ds = false;  // counted in ds->server_refs until done or failed
   double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
   int32_t rounds = 0, preemptions = 0;
   int64_t pp = 0;
@@ -111,11 +117,14 @@ struct ServerReq {
 // ~2e-6 rad for the base phase and the hardware sin/cos.  The sums' errors are partly correlated
 // along the aperture and dark pixels weigh in the RMS, so the measured RMS error is 0.2-2.0 times
 // this model (72 requests on grids 1-6 m apart, Sint 400-3500); the estimate is the model times 3.
+// The range series' truncation adds kabs x split_truncation_m (a systematic per-term error).
 double split_phase_estimate(const sarbp_dataset *ds, const SplitRows &rows) {
   if (!rows.any) return 0.0;
-  if (rows.omax > 48.0) return 1e9;  // rows this wide take FP64 terms in the kernel anyway
   const double kabs = std::fabs(-4.0 * SARBP_PI32_F64 / ds->lambda_val);
-  return 3.0 * (kabs * 1.2e-7 * rows.amax + 2e-6);
+  const double trunc = kabs * split_truncation_m(rows.omax, rows.r0min);
+  // rows this wide, or this close to the track, take FP64 terms in the kernel anyway
+  if (rows.omax > (double)SPLIT_OMAX_M || trunc > SPLIT_TRUNC_MAX_RAD) return 1e9;
+  return 3.0 * (kabs * 1.2e-7 * rows.amax + 2e-6) + trunc;
 }
 
 }  // namespace
@@ -155,15 +164,25 @@ struct sarbp_server {
     r->on_device = false;
     bytes_in_use -= r->need;
   }
+  static void drop_saved(ServerReq *r) {
+    std::vector<BpState>().swap(r->saved);
+    std::vector<double2>().swap(r->saved_out);
+    std::vector<int32_t>().swap(r->saved_cnt);
+  }
   // A finished (done or failed) request keeps only its record: its host copy of the pixel arrays
   // and any saved state are dropped, so a long-running server holds the pixels of the requests in
   // flight only.
+  static void drop_ds(ServerReq *r) {
+    if (r->holds_ds) r->ds->server_refs.fetch_sub(1);
+    r->holds_ds = false;
+  }
   void finish(ServerReq *r, int32_t state) {
     r->state = state;
+    drop_ds(r);
     hpool.put(r->host, r->host_cap);  // no upload from it is pending: the request left the device
     r->host = nullptr;
     r->host_cap = 0;
-    std::vector<BpState>().swap(r->saved);
+    drop_saved(r);
     std::vector<int>().swap(r->tile_len);
     r->req.x = r->req.y = r->req.z = nullptr;
     r->req.mask = nullptr;
@@ -174,7 +193,8 @@ struct sarbp_server {
 bool sarbp_server::admit(ServerReq *r) {
   // the uploads are ordered before the round's launch on the same stream; only a saved state,
   // whose host copy is dropped below, needs the stream drained first
-  int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len);
+  int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len,
+                         r->r0min);
   if (rc) return false;
   if (r->nseg > 1 && r->buf.n) {  // segment slots of tiles shorter than the last segment stay zero
     hipError_t e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st);
@@ -184,10 +204,14 @@ bool sarbp_server::admit(ServerReq *r) {
       return false;
     }
   }
-  if (r->off == 0) std::vector<BpState>().swap(r->saved);  // no progress to restore
+  if (r->off == 0) drop_saved(r);  // no progress to restore
   if (!r->saved.empty()) {
-    hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), r->saved.size() * sizeof(BpState),
-                                  hipMemcpyHostToDevice, st);
+    // the partial sums of unfinished tiles, and the results of the tiles that finished before the
+    // preemption (the round loop never schedules those again)
+    const size_t n = r->saved.size();
+    hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), n * sizeof(BpState), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->buf.out, r->saved_out.data(), n * sizeof(double2), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(r->buf.ncnt, r->saved_cnt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st);
@@ -195,7 +219,7 @@ bool sarbp_server::admit(ServerReq *r) {
       return false;
     }
   }
-  std::vector<BpState>().swap(r->saved);
+  drop_saved(r);
   r->on_device = true;
   r->max_len = 0;
   for (int v : r->buf.tile_len) r->max_len = std::max(r->max_len, v);
@@ -235,11 +259,19 @@ void sarbp_server::run() {
         ++it;
         continue;
       }
-      r->saved.resize((size_t)r->buf.n);
+      const size_t n = (size_t)r->buf.n;
+      r->saved.resize(n);
+      r->saved_out.resize(n);
+      r->saved_cnt.resize(n);
       lk.unlock();
-      hipError_t e = r->buf.n ? hipMemcpyAsync(r->saved.data(), r->buf.state, r->saved.size() * sizeof(BpState),
-                                               hipMemcpyDeviceToHost, st)
-                              : hipSuccess;
+      hipError_t e = hipSuccess;
+      if (n) {
+        e = hipMemcpyAsync(r->saved.data(), r->buf.state, n * sizeof(BpState), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(r->saved_out.data(), r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+          e = hipMemcpyAsync(r->saved_cnt.data(), r->buf.ncnt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
+      }
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       lk.lock();
       release(r);
@@ -400,6 +432,14 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
     delete s;
     return sarbp_fail(SARBP_EINVAL, "pulse_chunk must be a multiple of 8, at most 1024");
   }
+  {
+    const int rows = 64 / default_tiling(SARBP_PREC_SPLIT).tile_x;  // rows of the default split tile
+    if (sarbp_batch_lds_bytes(1, c.pulse_chunk, rows) > SARBP_BATCH_LDS_MAX) {
+      delete s;
+      return sarbp_fail(SARBP_EINVAL, "pulse_chunk too large: its records for the default split tile's rows "
+                                      "exceed the batch kernel's 64 KiB of LDS");
+    }
+  }
   s->cfg = c;
   hipError_t e = hipGetDevice(&s->device);
   int ncu = 0;
@@ -453,6 +493,7 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   r->req = *req;  // the scheduling decisions below read the caller's arrays
   r->precision = req->precision;
   r->req.tile_x = tx;
+  int split_rows = 0;  // rows per tile of a request that runs split (its LDS records per pulse)
   {
     TileGrid tg;
     SplitRows rows;
@@ -460,14 +501,22 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
     request_tiles(ds, r->req, tg, r->tile_len, r->pp, split ? &rows : nullptr);
     if (split) {
       r->est = split_phase_estimate(ds, rows);
+      r->r0min = (float)rows.r0min;
       if (r->est > s->cfg.phase_err_limit_rad) {  // FP64 terms, on the FP64 tiling
         r->precision = SARBP_PREC_F64;
         r->req.precision = SARBP_PREC_F64;
         r->req.tile_x = 0;
         request_tiles(ds, r->req, tg, r->tile_len, r->pp);
+      } else {
+        split_rows = tg.ty;
       }
     }
   }
+  // a split request's records of one pulse chunk must fit the batch kernel's LDS; checked here so
+  // that the request is refused alone instead of failing the round it would join
+  if (split_rows > 0 && sarbp_batch_lds_bytes(1, s->cfg.pulse_chunk, split_rows) > SARBP_BATCH_LDS_MAX)
+    return sarbp_fail(SARBP_EINVAL, "split request: pulse_chunk x tile rows x 48 B exceeds the batch kernel's "
+                                    "64 KiB of LDS (use a wider tile_x or a smaller pulse_chunk)");
   {
     int max_len = 0;
     int64_t live = 0;
@@ -514,6 +563,8 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   }
   r->id = s->next_id++;
   r->submit_s = t_entry;
+  r->holds_ds = true;
+  ds->server_refs.fetch_add(1);
   *id = r->id;
   s->queue.push_back(r.get());
   s->reqs[r->id] = std::move(r);
@@ -642,8 +693,10 @@ int sarbp_server_destroy(sarbp_server *s) {
   if (s->st) (void)hipStreamSynchronize(s->st);
   s->ib.release();
   s->pool.release();
-  for (auto &kv : s->reqs)  // preempted requests never resumed still hold their host copy
+  for (auto &kv : s->reqs) {  // preempted requests never resumed still hold their host copy
     if (kv.second->host) (void)hipHostFree(kv.second->host);
+    sarbp_server::drop_ds(kv.second.get());
+  }
   s->hpool.release();
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);

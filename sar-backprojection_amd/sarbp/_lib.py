@@ -79,6 +79,7 @@ SIGNATURES = {
     "sarbp_dataset_bytes": (ctypes.c_int, [_p, ctypes.POINTER(_i64)]),
     "sarbp_scene_create": (ctypes.c_int, [_p, _p, _p, _p, _p, _p, _p, _i64, _i64, _pp]),
     "sarbp_scene_set_tiling": (ctypes.c_int, [_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "sarbp_scene_set_bin_min": (ctypes.c_int, [_p, _i32]),
     "sarbp_scene_destroy": (ctypes.c_int, [_p]),
     "sarbp_scene_focus": (ctypes.c_int, [_p, _p, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
     "sarbp_scene_download": (ctypes.c_int, [_p, _p, _p]),

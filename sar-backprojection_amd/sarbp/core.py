@@ -103,9 +103,11 @@ class Dataset:
         return int(v.value)
 
     def close(self):
+        """Release the device memory.  Raises SarbpError (ESTATE) while a server still has requests
+        against this dataset that are not done or failed; the dataset then stays valid."""
         h = getattr(self, "_h", None)
         if h:
-            _lib.load().sarbp_dataset_destroy(h)
+            check(_lib.load().sarbp_dataset_destroy(h), "sarbp_dataset_destroy")
             self._h = None
 
     def __del__(self):
@@ -198,6 +200,10 @@ class Scene:
         """Work mapping for one precision (sarbp_scene_set_tiling)."""
         check(_lib.load().sarbp_scene_set_tiling(self._h, precision_code(precision), int(kernel), int(tile_x),
                                                   int(accum)), "sarbp_scene_set_tiling")
+
+    def set_bin_min(self, bin_min: int):
+        """Lowest contributing range bin, process_point's ind_line_ra_min (sarbp_scene_set_bin_min)."""
+        check(_lib.load().sarbp_scene_set_bin_min(self._h, int(bin_min)), "sarbp_scene_set_bin_min")
 
     def count_pp(self) -> int:
         return count_pp(self.mask, self.index, self.sint, self.ds.naz)

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 from . import formats
-from .sar_focuser_puntual import _dataset_for, _flatten_inputs
+from .core import Dataset
+from .sar_focuser_puntual import _flatten_inputs
 
 
 def _check_2d(Raw_Data, xxg1):
@@ -28,8 +29,8 @@ def FocalizzatoreBpCuPy(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz, Raw_Dat
     _check_2d(Raw_Data, xxg1)
     dimx, dimy = np.shape(xxg1)
     xf, yf, zf, mf, inf, sf = _flatten_inputs(xxg1, yyg1, zzg1, mask, index, Sint)
-    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
-    out = ds.focus_flat(xf, yf, zf, mf, inf, sf, precision, dimx=dimx)
+    with Dataset(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val) as ds:  # the arrays as they are now
+        out = ds.focus_flat(xf, yf, zf, mf, inf, sf, precision, dimx=dimx)
     return np.reshape(out, (dimx, dimy), order="F")
 
 
@@ -43,12 +44,12 @@ def FocalizzatoreBpCuPy_Batched(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz,
     dimx, dimy = np.shape(xxg1)
     dim = dimx * dimy
     xf, yf, zf, mf, inf, sf = _flatten_inputs(xxg1, yyg1, zzg1, mask, index, Sint)
-    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
     step = -(-int(batch_size) // dimx) * dimx
     out = np.zeros(dim, np.complex128)
-    for i in range(0, dim, step):
-        j = min(dim, i + step)
-        out[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision, dimx=dimx)
+    with Dataset(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val) as ds:  # RawData resident once
+        for i in range(0, dim, step):
+            j = min(dim, i + step)
+            out[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision, dimx=dimx)
     return np.reshape(out, (dimx, dimy), order="F")
 
 

@@ -1,4 +1,5 @@
-"""Counter calibration: time a plain device copy of a known size (libsarbp_probe.so).
+"""Counter calibration: time a plain device copy of a known size (libsarbp_probe.so, a measurement
+tool built apart from the library: make -C sar-backprojection_amd/csrc probe).
 
     python -m sarbp.probe --mib 1024 --reps 5
 
@@ -15,6 +16,8 @@ _SO = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsarbp_probe.s
 
 
 def copy(bytes_: int, reps: int = 5) -> float:
+    if not os.path.exists(_SO):
+        raise ImportError(f"{_SO} is not built: make -C sar-backprojection_amd/csrc probe")
     L = ctypes.CDLL(_SO)
     L.sarbp_probe_copy.argtypes = [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
     L.sarbp_probe_copy.restype = ctypes.c_int

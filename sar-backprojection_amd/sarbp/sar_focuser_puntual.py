@@ -6,8 +6,8 @@ libsarbp's FP64 HIP kernel instead of the Numba loop.
 """
 from __future__ import annotations
 
+import hashlib
 import os
-import weakref
 
 import numpy as np
 
@@ -16,45 +16,63 @@ from .core import Dataset, precision_code
 
 CHECKPOINT_FILE = "focalizador_px_checkpoint.npz"  # sar_focuser_puntual.py:218
 
-_cache: dict = {}
+_point_cache: dict = {}
 
 
-def _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val) -> Dataset:
-    """Device dataset for these arrays, reused while the same host buffers are passed again
-    (process_point is called once per pixel by the reference's loop)."""
+def _digest(Raw_Data, wgsx, wgsy, wgsz) -> bytes:
+    """Content check of the arrays behind a cached device dataset: the whole trajectory and 65536
+    evenly spaced RawData samples."""
+    raw = np.asarray(Raw_Data)
+    h = hashlib.blake2b(digest_size=16)
+    for t in (wgsx, wgsy, wgsz):
+        h.update(np.ascontiguousarray(t, dtype=np.float64).tobytes())
+    step = max(1, raw.size // 65536)
+    h.update(np.ascontiguousarray(raw.reshape(-1, order="A")[::step]).tobytes())
+    return h.digest()
+
+
+def _point_dataset(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val) -> Dataset:
+    """Device dataset for process_point, which the reference's loop calls once per pixel with the
+    same arrays: kept resident while the same buffers with the same content (trajectory in full,
+    a strided sample of RawData) are passed again."""
     raw = np.asarray(Raw_Data)
     key = (raw.__array_interface__["data"][0], raw.shape, raw.strides, str(raw.dtype),
            tuple(np.asarray(t).__array_interface__["data"][0] for t in (wgsx, wgsy, wgsz)),
            float(near_range), float(dra), float(lambda_val))
-    ent = _cache.get(key)
-    if ent is not None:
-        ref, ds = ent
-        if ref() is raw or ref() is Raw_Data:
-            return ds
+    dig = _digest(Raw_Data, wgsx, wgsy, wgsz)
+    ent = _point_cache.get(key)
+    if ent is not None and ent[0] == dig:
+        return ent[1]
+    for _, ds_old in _point_cache.values():
+        ds_old.close()
+    _point_cache.clear()  # at most one resident dataset for this entry point
     ds = Dataset(raw, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
-    _cache.clear()  # keep at most one resident dataset for the compatibility entry points
-    try:
-        ref = weakref.ref(Raw_Data)
-    except TypeError:
-        ref = lambda: Raw_Data  # noqa: E731
-    _cache[key] = (ref, ds)
+    _point_cache[key] = (dig, ds)
     return ds
 
 
 def process_point(xxg1_flat_ii, yyg1_flat_ii, zzg1_flat_ii, mask_flat_ii, index_flat_ii, Sint_flat_ii,
                   wgsx, wgsy, wgsz, Raw_Data, near_range, dra, lambda_val, Nra, Naz,
                   ind_line_ra_min=0, index_min=0):
-    """One pixel (sar_focuser_puntual.py:21-155); returns complex128."""
-    if ind_line_ra_min != 0 or index_min != 0:
-        raise NotImplementedError("only ind_line_ra_min = index_min = 0 is supported "
-                                  "(the reference always passes 0, sar_focuser_puntual.py:214-215)")
+    """One pixel (sar_focuser_puntual.py:21-155); returns complex128.
+
+    ``ind_line_ra_min`` is the lowest range bin kept (:106; negative bins are skipped by the
+    accumulation loop, :138).  ``index_min`` is accepted and, as in the reference, unused: the
+    trajectory is indexed with the absolute pulse numbers (:76-78)."""
     raw = np.asarray(Raw_Data)
     if raw.ndim != 2 or raw.shape != (int(Nra), int(Naz)):
         raise ValueError(f"Raw_Data shape {raw.shape} does not match Nra={Nra}, Naz={Naz}")
-    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
-    out = ds.focus_flat(np.array([xxg1_flat_ii], np.float64), np.array([yyg1_flat_ii], np.float64),
-                        np.array([zzg1_flat_ii], np.float64), np.array([mask_flat_ii], np.uint8),
-                        np.array([index_flat_ii], np.int32), np.array([Sint_flat_ii], np.int32))
+    ds = _point_dataset(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
+    one = lambda v, dt: np.array([v], dt)  # noqa: E731
+    sc = ds.scene(one(xxg1_flat_ii, np.float64), one(yyg1_flat_ii, np.float64), one(zzg1_flat_ii, np.float64),
+                  one(mask_flat_ii, np.uint8), one(index_flat_ii, np.int32), one(Sint_flat_ii, np.int32))
+    try:
+        if int(ind_line_ra_min) != 0:
+            sc.set_bin_min(int(ind_line_ra_min))
+        sc.focus("f64")
+        out = sc.result()
+    finally:
+        sc.close()
     return np.complex128(out[0])
 
 
@@ -90,18 +108,19 @@ def FocalizzatoreBpPx(xxg1, yyg1, zzg1, mask, index, wgsx, wgsy, wgsz, Raw_Data,
         ck = formats.load_px_checkpoint(checkpoint_file, dim)
         if ck is not None:
             focused, start = ck
-    ds = _dataset_for(Raw_Data, wgsx, wgsy, wgsz, near_range, dra, lambda_val)
-    # chunks of whole grid columns keep the tiles two-dimensional
-    step = dim - start if not checkpoint_every else -(-int(checkpoint_every) // dimx) * dimx
-    i = start
-    while i < dim:
-        j = min(dim, i + max(step, 1))
-        cols = dimx if (i % dimx == 0 and (j - i) % dimx == 0) else 0
-        focused[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision,
-                                     dimx=cols)
-        i = j
-        if checkpoint_file and checkpoint_every and i < dim:
-            formats.save_px_checkpoint(checkpoint_file, focused, i)
+    # the arrays as they are now (a caller may change them in place between calls)
+    with Dataset(raw, wgsx, wgsy, wgsz, near_range, dra, lambda_val) as ds:
+        # chunks of whole grid columns keep the tiles two-dimensional
+        step = dim - start if not checkpoint_every else -(-int(checkpoint_every) // dimx) * dimx
+        i = start
+        while i < dim:
+            j = min(dim, i + max(step, 1))
+            cols = dimx if (i % dimx == 0 and (j - i) % dimx == 0) else 0
+            focused[i:j] = ds.focus_flat(xf[i:j], yf[i:j], zf[i:j], mf[i:j], inf[i:j], sf[i:j], precision,
+                                         dimx=cols)
+            i = j
+            if checkpoint_file and checkpoint_every and i < dim:
+                formats.save_px_checkpoint(checkpoint_file, focused, i)
     if checkpoint_file and os.path.exists(checkpoint_file):
         try:
             os.remove(checkpoint_file)

@@ -25,10 +25,13 @@ from .batch import Request
 
 
 class Handle:
-    """A submitted request: keeps the result buffers alive until completion."""
+    """A submitted request.  It holds the dataset, the request's arrays and the result buffers the
+    library writes into; the server keeps every handle of an unfinished request alive, so dropping
+    one before completion is safe."""
 
-    def __init__(self, server, rid, req: Request, out, cnt, t_submit):
-        self.server, self.id, self.req, self.out, self.cnt, self.t_submit = server, rid, req, out, cnt, t_submit
+    def __init__(self, server, rid, ds, req: Request, out, cnt, t_submit):
+        self.server, self.id, self.ds, self.req = server, rid, ds, req
+        self.out, self.cnt, self.t_submit = out, cnt, t_submit
 
     def wait(self, timeout=None) -> bool:
         """True when the request is done; False on timeout or when it was preempted (see state());
@@ -38,12 +41,15 @@ class Handle:
             return False
         if st == _lib.ESTATE and self.state() == _lib.REQ_PREEMPTED:
             return False
+        self.server._settled(self.id)
         check(st, "sarbp_server_wait")
         return True
 
     def state(self) -> int:
         v = ctypes.c_int32()
         check(_lib.load().sarbp_server_poll(self.server._h, self.id, ctypes.byref(v)), "sarbp_server_poll")
+        if v.value in (_lib.REQ_DONE, _lib.REQ_FAILED):
+            self.server._settled(self.id)
         return int(v.value)
 
     def info(self) -> dict:
@@ -75,6 +81,8 @@ class Server:
         h = ctypes.c_void_p()
         check(_lib.load().sarbp_server_create(ctypes.byref(cfg), ctypes.byref(h)), "sarbp_server_create")
         self._h = h
+        self._inflight = {}  # id -> Handle of every request not yet seen done or failed
+        self._since_reap = 0
         _lib.track(self)
 
     def submit(self, ds, req: Request) -> Handle:
@@ -85,7 +93,27 @@ class Server:
         t = time.perf_counter()
         check(_lib.load().sarbp_server_submit(self._h, ds.handle, ctypes.byref(st), out.ctypes.data, cnt.ctypes.data,
                                               ctypes.byref(rid)), "sarbp_server_submit")
-        return Handle(self, int(rid.value), req, out, cnt, t)
+        hd = Handle(self, int(rid.value), ds, req, out, cnt, t)
+        self._inflight[hd.id] = hd
+        self._since_reap += 1
+        if self._since_reap >= 64:
+            self.reap()
+        return hd
+
+    def _settled(self, rid):
+        self._inflight.pop(rid, None)
+
+    def reap(self) -> int:
+        """Release the server's references to requests that have finished (done or failed);
+        returns the number still in flight."""
+        self._since_reap = 0
+        L = _lib.load()
+        v = ctypes.c_int32()
+        for rid in list(self._inflight):
+            if L.sarbp_server_poll(self._h, rid, ctypes.byref(v)) == _lib.OK and v.value in (_lib.REQ_DONE,
+                                                                                            _lib.REQ_FAILED):
+                self._inflight.pop(rid, None)
+        return len(self._inflight)
 
     def stats(self) -> dict:
         s = _lib.ServerStats()
@@ -93,10 +121,12 @@ class Server:
         return {k: getattr(s, k) for k, _ in _lib.ServerStats._fields_}
 
     def close(self):
+        """Finish the current round, fail what is still queued or running, stop the server."""
         h = getattr(self, "_h", None)
         if h:
             _lib.load().sarbp_server_destroy(h)
             self._h = None
+        self._inflight = {}  # the library no longer writes into any request's buffers
 
     def __del__(self):
         try:  # at interpreter exit the module's globals may already be gone
@@ -112,25 +142,32 @@ def percentile(v, q):
 
 def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: float, seed: int = 0,
                 warmup_s: float = 2.0, log=None):
-    """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at Poisson times of
-    rate ``rate_hz`` for ``duration_s``; wait for all; latency = completion - arrival on the server
-    clock, arrival being the entry to the library's submit call (queueing included).
+    """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at the times of a seeded
+    Poisson schedule of rate ``rate_hz`` for ``duration_s``; wait for all.
+
+    Latency is completion minus ARRIVAL, the request's time in the Poisson schedule: it includes
+    any delay before the driver got to submit it (a driver that falls behind the schedule shows up
+    as latency), the submission itself and the queueing.  Completion times are on the server's clock
+    (``done_s``); the driver's clock is mapped onto it by the smallest observed difference between
+    a request's ``submit_s`` (server clock, entry to submit) and the driver's time just before the
+    call.  The same statistics from the entry to submit are reported as ``submit_latency_*``.
     Finished requests are harvested as the run goes (their result buffers released), so a long run
     holds only the requests in flight."""
     rng = np.random.default_rng(seed)
-    pending = collections.deque()  # (arrival offset, Handle), in submission order
-    done = []                      # (arrival offset, info)
+    pending = collections.deque()  # (scheduled arrival offset, driver time at the call, Handle)
+    done = []                      # (scheduled arrival offset, driver time at the call, info)
 
     def harvest(block):
         while pending:
-            t_arr, h = pending[0]
+            t_arr, t_call, h = pending[0]
             if block:
                 h.wait()
             elif h.state() not in (_lib.REQ_DONE, _lib.REQ_FAILED):
                 break
-            done.append((t_arr, h.info()))
+            done.append((t_arr, t_call, h.info()))
             pending.popleft()
 
+    st0 = server.stats()
     t0 = time.perf_counter()
     t_next = 0.0
     k = 0
@@ -142,7 +179,8 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
             time.sleep(min(t_next - now, 0.05))
             continue
         di, req = pool[k % len(pool)]
-        pending.append((t_next, server.submit(datasets[di], req)))
+        t_call = time.perf_counter()
+        pending.append((t_next, t_call, server.submit(datasets[di], req)))
         k += 1
         t_next += rng.exponential(1.0 / rate_hz)
         if log and time.perf_counter() - last_log > 10:
@@ -150,23 +188,34 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
             log(f"rate {rate_hz}/s: {k} submitted, {len(pending)} in flight, t={now:.0f}s, {server.stats()}")
     harvest(True)
     t_end = time.perf_counter() - t0
-    lat, pp, prec = [], 0, {"f64": 0, "split": 0}
-    first_submit, last_done = float("inf"), 0.0
-    for t_arr, inf in done:
+    st1 = server.stats()
+    # driver clock -> server clock
+    offset = min((inf["submit_s"] - t_call for _, t_call, inf in done), default=0.0)
+    lat, lat_submit, lag, pp, prec = [], [], [], 0, {"f64": 0, "split": 0}
+    first_submit, last_done = float("inf"), float("-inf")
+    for t_arr, t_call, inf in done:
         if t_arr >= warmup_s:
-            lat.append(inf["done_s"] - inf["submit_s"])
+            lat.append(inf["done_s"] - (t0 + t_arr + offset))
+            lat_submit.append(inf["done_s"] - inf["submit_s"])
+            lag.append(t_call - (t0 + t_arr))
         pp += inf["pp"]
         first_submit = min(first_submit, inf["submit_s"])
         last_done = max(last_done, inf["done_s"])
         prec["split" if inf["precision"] == _lib.PREC_SPLIT else "f64"] += 1
-    span = max(1e-9, last_done - first_submit)
+    span = max(1e-9, last_done - first_submit) if done else 1e-9
+    ms = lambda v, q: percentile(v, q) * 1e3 if v else None  # noqa: E731
+    rounds = st1.get("rounds", 0) - st0.get("rounds", 0)
+    busy = st1.get("gpu_busy_s", 0.0) - st0.get("gpu_busy_s", 0.0)
     return {
         "rate_req_per_s": rate_hz, "duration_s": duration_s, "requests": len(done),
-        "latency_p50_ms": percentile(lat, 50) * 1e3 if lat else None,
-        "latency_p99_ms": percentile(lat, 99) * 1e3 if lat else None,
+        "latency_p50_ms": ms(lat, 50), "latency_p99_ms": ms(lat, 99),
         "latency_max_ms": max(lat) * 1e3 if lat else None,
+        "submit_latency_p50_ms": ms(lat_submit, 50), "submit_latency_p99_ms": ms(lat_submit, 99),
+        "arrival_lag_p99_ms": ms(lag, 99),
         "pp_total": int(pp), "pp_per_s": pp / span, "wall_s": t_end, "precision_counts": prec,
-        "server": server.stats(),
+        "rounds": int(rounds), "rounds_per_request": rounds / max(1, len(done)),
+        "gpu_busy_frac": busy / span,
+        "server": st1,
     }
 
 

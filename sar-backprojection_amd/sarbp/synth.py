@@ -233,3 +233,50 @@ def tiny_scene(seed: int = 2, nx: int = 24, ny: int = 12, naz: int = 801, nra: i
     raw = _simulate_raw(rng, targets, tx, ty, tz, nra, near, dra, lam, snr_db)
     return Scene(np.asfortranarray(X), np.asfortranarray(Y), np.asfortranarray(Z), mask, index, sint,
                  tx, ty, tz, raw, near, dra, lam, targets, f"tiny-{nx}x{ny}")
+
+
+EDGE_SINTS = tuple(range(-5, 13)) + (-9999, 400, 2000)
+
+
+def edge_grid_scene(seed: int = 3, naz: int = 4001, nra: int = 64, near: float = 4300.0, dra: float = 0.5,
+                    lam: float = 0.031, end: bool = False) -> Scene:
+    """Every aperture and range-bin edge case on one grid: columns take the aperture centres
+    Index = -3..3 (``end=False``) or Naz-4..Naz+2 (``end=True``), rows the Sint values
+    ``EDGE_SINTS`` (-5..12, -9999, and two long apertures, 400 and 2000 pulses, the second with a
+    range migration that crosses a bin edge inside the aperture), in three bands of rows whose
+    broadside range bins step by 0.3 bins across bin 0, across the last bin Nra-1 (excluded by the
+    reference) and inside the swath.  Pixels 1 m apart along the track near its start (or end), z = 0; track along x at 0.1 m
+    with 2 cm jitter, y = -3000 m, z = 3000 m; RawData is seeded complex Gaussian noise.  Grid shape
+    (7, 3 * len(EDGE_SINTS)), F order."""
+    rng = np.random.default_rng(seed)
+    tx = np.arange(naz) * 0.1 + rng.normal(0, 0.02, naz)
+    ty = np.full(naz, -3000.0) + rng.normal(0, 0.02, naz)
+    tz = np.full(naz, 3000.0) + rng.normal(0, 0.02, naz)
+    idx = np.arange(naz - 4, naz + 3) if end else np.arange(-3, 4)
+    x0 = (naz - 1) * 0.1 - 6.0 if end else 0.0
+    ns = len(EDGE_SINTS)
+    # per band, the bin edge it straddles: bin 0 (the first that counts), Nra-1 (the first that does
+    # not), and the swath's middle.  Short apertures step across the edge by 0.3 bins from edge-2.4;
+    # the 400-pulse aperture sits 0.4 bins below it, the 2000-pulse one 1.4 bins below, so that its
+    # range migration (~2.3 bins over 100 m of track) carries it across the edge.
+    edges = (0.0, nra - 1.0, nra / 2.0)
+    offset = {400: -0.4, 2000: -1.4}
+    nx, ny = idx.size, 3 * ns
+    X = np.empty((nx, ny))
+    Y = np.empty((nx, ny))
+    index = np.empty((nx, ny), np.int32)
+    sint = np.empty((nx, ny), np.int32)
+    for b, e in enumerate(edges):
+        for r, s in enumerate(EDGE_SINTS):
+            j = b * ns + r
+            R = near + (e + offset.get(s, -2.4 + 0.3 * r)) * dra
+            Y[:, j] = np.sqrt(R * R - 3000.0 ** 2) - 3000.0
+            X[:, j] = x0 + np.arange(nx)
+            index[:, j] = idx
+            sint[:, j] = s
+    Z = np.zeros((nx, ny))
+    mask = np.ones((nx, ny), np.uint8)
+    raw = np.asfortranarray((rng.standard_normal((nra, naz)) + 1j * rng.standard_normal((nra, naz))).astype(np.complex64))
+    f = np.asfortranarray
+    return Scene(f(X), f(Y), f(Z), f(mask), f(index), f(sint), tx, ty, tz, raw, near, dra, lam, None,
+                 f"edge-grid-{'end' if end else 'start'}")

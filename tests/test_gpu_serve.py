@@ -197,3 +197,90 @@ def test_split_estimate_bounds_measured_phase_error(gpu, world):
         assert e["phase_rms_rad_rel1e-5max"] <= est
         assert e["phase_rms_rad_rel1e-5max"] <= 1e-3
     srv.close()
+
+
+def test_preempt_after_short_tiles_finished(gpu, world):
+    """Tiles of unequal union length: the short ones finish (and write their means) rounds before the
+    long ones.  Preempted after that and resumed, the request must still return every pixel, equal
+    to the uninterrupted run."""
+    sc, ds, _ = world
+    # 256 x 160 px (enough pixels that a round's pulse span is one 8-pulse chunk): the left half
+    # with 300-pulse apertures (tile unions ~450 pulses, done after ~60 rounds), the right half with
+    # 4000-pulse ones (~520 rounds)
+    sub = sc.sub(100, 356, 40, 200)
+    f = lambda a: np.asarray(a).flatten(order="F")  # noqa: E731
+    sint = np.where(np.arange(256)[:, None] < 128, 300, 4000) * np.ones((1, 160), np.int64)
+    for prec in ("split", "f64"):
+        req = Request(f(sub.x), f(sub.y), f(sub.z), f(sub.mask), f(sub.index), f(sint.astype(np.int32)), dimx=256,
+                      precision=prec)
+        srv = Server(quantum_ms=0.001, pulse_chunk=8, max_segments=1)  # 8 pulses per round
+        ref = srv.submit(ds, req)
+        assert ref.wait(timeout=120)
+        h = srv.submit(ds, req)
+        t0 = time.time()
+        while h.info()["rounds"] < 70 and time.time() - t0 < 60:
+            time.sleep(0.0002)
+        h.preempt()
+        assert not h.wait(timeout=60)
+        inf = h.info()
+        print("case preempt after short tiles,", prec, inf, flush=True)
+        assert inf["state"] == _lib.REQ_PREEMPTED and 60 < inf["rounds"] < 400
+        h.resume()
+        assert h.wait(timeout=120)
+        got, cnt = h.result(with_counts=True)
+        want, wcnt = ref.result(with_counts=True)
+        assert np.array_equal(cnt, wcnt) and np.array_equal(got, want), prec
+        assert np.all(cnt[f(sint) == 300] > 0)
+        srv.close()
+
+
+def test_request_lifetime(gpu, world):
+    """The server keeps a request's buffers and dataset alive until it is done, even when the caller
+    drops its handle; a dataset with unfinished server requests refuses to be destroyed."""
+    import gc
+    sc, _, _ = world
+    own = Dataset(sc.raw, sc.tx, sc.ty, sc.tz, sc.near, sc.dra, sc.lam)
+    reqs = patch_requests(sc, 3, seed=11, w_range=(64, 128), h_range=(32, 64), sint_range=(2000, 4000))
+    srv = Server(quantum_ms=0.001, pulse_chunk=8, max_segments=1)
+    hs = [srv.submit(own, r) for r in reqs]
+    hs[-1].preempt()  # preempted (queued or running): it may be resumed, so it holds the dataset
+    assert not hs[-1].wait(timeout=60)
+    with pytest.raises(_lib.SarbpError) as ei:
+        own.close()
+    assert ei.value.status == _lib.ESTATE
+    rid = hs[0].id
+    del hs[0]  # the caller drops a handle before completion
+    gc.collect()
+    for h in hs[:-1]:
+        assert h.wait(timeout=120)
+    while srv.reap():
+        time.sleep(0.01)
+        if srv._inflight and all(h.state() == _lib.REQ_PREEMPTED for h in srv._inflight.values()):
+            break
+    assert list(srv._inflight) == [hs[-1].id] and rid not in srv._inflight
+    with pytest.raises(_lib.SarbpError):
+        own.close()
+    hs[-1].resume()
+    assert hs[-1].wait(timeout=120)
+    own.close()  # every request is done: the dataset goes
+    srv.close()
+
+
+def test_split_request_beyond_lds_is_refused_alone(gpu, world):
+    """A split request whose tile rows x pulse chunk exceed the batch kernel's LDS is refused at
+    submission; the other requests run; a server whose chunk cannot hold the default split tile is
+    refused at creation."""
+    sc, ds, _ = world
+    with pytest.raises(ValueError):
+        Server(pulse_chunk=512)  # 512 pulses x 4 rows x 48 B > 64 KiB
+    srv = Server(pulse_chunk=256)  # 4 rows: 48 KiB; 16 rows (tile_x 4): 192 KiB
+    good = patch_requests(sc, 2, seed=12, w_range=(32, 64), h_range=(16, 32))
+    tall = Request(good[0].x, good[0].y, good[0].z, good[0].mask, good[0].index, good[0].sint, dimx=good[0].dimx,
+                   precision="split", tile_x=4)
+    with pytest.raises(ValueError):
+        srv.submit(ds, tall)
+    hs = [srv.submit(ds, r) for r in good]
+    for h in hs:
+        assert h.wait(timeout=60)
+    assert srv.stats()["failed"] == 0
+    srv.close()

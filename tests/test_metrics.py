@@ -115,6 +115,20 @@ def test_split_error_report_known_errors():
         assert r[k] == pytest.approx(1e-4, rel=1e-6), k
     assert r["amp_rel_rmse"] == pytest.approx(1e-5, rel=1e-6)
     assert r["amp_rel_max_rel1e-5max"] == pytest.approx(1e-5, rel=1e-6)
+    for k in ("phase_rms_rad_rel1e-2max", "phase_max_rad_rel1e-2max"):
+        assert r[k] == pytest.approx(1e-4, rel=1e-6), k
+    assert r["amp_rel_rms_rel1e-2max"] == pytest.approx(1e-5, rel=1e-6)
+    assert r["amp_rel_max_rel1e-2max"] == pytest.approx(1e-5, rel=1e-6)
+    a = np.abs(ref)
+    assert r["pixels_rel1e-2max"] == int((a > 1e-2 * a.max()).sum()) > 0
+    # an error planted on one dark pixel (1e-3 x its amplitude) shows in the 1e-5 x max tails only
+    dark_ref = ref.copy()
+    dark_ref.flat[7] *= 1e-3
+    bad = dark_ref.copy()
+    bad.flat[7] *= 1.5 * np.exp(0.5j)
+    r2 = metrics.split_error_report(bad, dark_ref)
+    assert r2["phase_max_rad_rel1e-5max"] == pytest.approx(0.5) and r2["amp_rel_max_rel1e-5max"] == pytest.approx(0.5)
+    assert r2["phase_max_rad_rel1e-2max"] == 0.0 and r2["amp_rel_max_rel1e-2max"] == 0.0
     same = metrics.split_error_report(ref, ref)
     assert same["phase_rms_rad_abs1e-6"] == 0.0 and same["amp_rel_rmse"] == 0.0
 

@@ -1,7 +1,9 @@
 """Host side of the Poisson serving benchmark (sarbp.serve.poisson_run) with a stand-in server whose
-requests take a known time: arrivals follow the seeded schedule, every request is harvested once,
-the warm-up is excluded from the percentiles, and p50/p99/max, the pixel-pulse rate and the
-precision counts come out of the request records as defined (latency = done_s - submit_s).
+requests take a known time and whose clock has its own origin: arrivals follow the seeded schedule,
+every request is harvested once, the warm-up is excluded from the percentiles, the latency from the
+entry to submit is done_s - submit_s, the latency from the scheduled arrival maps the driver's clock
+onto the server's, and the pixel-pulse rate, round count and precision counts come out of the
+records as defined.
 """
 import numpy as np
 import pytest
@@ -21,16 +23,20 @@ class _Handle:
 
     def info(self):
         lat = self.server.latency(self.k)
-        return {"submit_s": self.t_submit, "done_s": self.t_submit + lat, "pp": 1000 + self.k,
+        c = self.server.CLOCK  # the server's clock starts elsewhere than the driver's
+        return {"submit_s": self.t_submit + c, "done_s": self.t_submit + c + lat, "pp": 1000 + self.k,
                 "precision": self.precision, "state": _lib.REQ_DONE}
 
 
 class _Server:
-    """Request k takes (k % 10 + 1) ms; it 'arrives' at the wall time of its submission."""
+    """Request k takes (k % 10 + 1) ms from the entry to submit; every 7th submission is delayed
+    by 3 ms before that entry (a driver behind its schedule)."""
+    CLOCK = -1234.5
 
     def __init__(self):
         self.n = 0
         self.t = []
+        self.calls = []
 
     @staticmethod
     def latency(k):
@@ -38,6 +44,9 @@ class _Server:
 
     def submit(self, ds, req):
         import time
+        self.calls.append(time.perf_counter())
+        if self.n % 7 == 3:
+            time.sleep(3e-3)
         t = time.perf_counter()
         self.t.append(t)
         h = _Handle(self, self.n, t, _lib.PREC_SPLIT if req == "s" else _lib.PREC_F64)
@@ -45,7 +54,7 @@ class _Server:
         return h
 
     def stats(self):
-        return {"submitted": self.n}
+        return {"submitted": self.n, "rounds": 2 * self.n, "gpu_busy_s": 1e-3 * self.n}
 
 
 def test_poisson_run_statistics():
@@ -61,14 +70,25 @@ def test_poisson_run_statistics():
         offs.append(t)
         t += rng.exponential(1.0 / 400.0)
     assert n == len(offs)
-    lat = [(k % 10 + 1) * 1e-3 for k in range(n) if offs[k] >= 0.1]
-    assert res["latency_p50_ms"] == pytest.approx(np.percentile(lat, 50) * 1e3)
-    assert res["latency_p99_ms"] == pytest.approx(np.percentile(lat, 99) * 1e3)
-    assert res["latency_max_ms"] == pytest.approx(max(lat) * 1e3)
+    keep = [k for k in range(n) if offs[k] >= 0.1]
+    lat = [(k % 10 + 1) * 1e-3 for k in keep]
+    assert res["submit_latency_p50_ms"] == pytest.approx(np.percentile(lat, 50) * 1e3)
+    assert res["submit_latency_p99_ms"] == pytest.approx(np.percentile(lat, 99) * 1e3)
+    # from the scheduled arrival: the stand-in's latency plus the time from the schedule to the entry
+    # to submit (the driver's own lag and the 3 ms delays); the clock offset is estimated from the
+    # fastest call, so each value is exact to within that call's overhead
+    t0 = srv.calls[0] - offs[0]  # the first request is called at its scheduled time (to within microseconds)
+    full = [srv.t[k] - (t0 + offs[k]) + (k % 10 + 1) * 1e-3 for k in keep]
+    assert res["latency_p50_ms"] == pytest.approx(np.percentile(full, 50) * 1e3, abs=0.3)
+    assert res["latency_p99_ms"] == pytest.approx(np.percentile(full, 99) * 1e3, abs=0.3)
+    assert res["latency_max_ms"] == pytest.approx(max(full) * 1e3, abs=0.3)
+    assert res["latency_max_ms"] >= 3.0 + 1.0  # a delayed submission is counted from its arrival
+    assert res["rounds"] == 2 * n and res["rounds_per_request"] == pytest.approx(2.0)
     assert res["precision_counts"] == {"split": (n + 1) // 2, "f64": n // 2}
     assert res["pp_total"] == sum(1000 + k for k in range(n))
     last_done = max(srv.t[k] + (k % 10 + 1) * 1e-3 for k in range(n))
     assert res["pp_per_s"] == pytest.approx(res["pp_total"] / (last_done - srv.t[0]), rel=1e-9)
+    assert res["gpu_busy_frac"] == pytest.approx(1e-3 * n / (last_done - srv.t[0]), rel=1e-9)
 
 
 def test_percentile_of_empty_is_none():

@@ -6,15 +6,23 @@ Per (pixel, pulse) the split kernels replace the slant-range difference |Pc + o 
 fourth-order term is used for offsets beyond SPLIT_FOURTH_M).  This checks that series, evaluated in
 FP64, against the exact difference over the geometry of the 4000 x 250 workload (slant ranges
 4.2-6 km, squint within +-25 degrees), with offsets in any direction up to the limits read from
-csrc/bp_tile.h: the truncation error, as phase (4 pi / lambda x error), stays below 2e-5 rad with
+csrc/sarbp_internal.h: the truncation error, as phase (4 pi / lambda x error), stays below 2e-5 rad with
 the second-order series up to SPLIT_FOURTH_M (measured 1.4e-5) and below 4e-5 rad with the
 fourth-order term up to SPLIT_OMAX_M (measured 2.7e-5) -- at worst of the order of the split
 form's FP32 rounding (~3e-5 rad RMS), and orders of magnitude below it for the along-track row
 offsets of a regular grid.  A dropped or mis-signed term, or a wrong coefficient, exceeds the same
 bounds twenty-fold or more (checked below on planted variants).
+
+The library's own bound of that truncation, split_truncation_m (csrc/sarbp_internal.h: the kernels
+take FP64 terms for a tile whose bound, as phase, exceeds 1e-4 rad, and widen the FP64 re-decision
+of near-tie bins by it; the server's fallback estimate adds it), is compiled here for the host and
+checked against the worst truncation found by sampling offset directions, from short slant ranges
+(300 m) to the workload's: it must bound every sampled case and stay within 2x of the worst one.
 """
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -24,8 +32,11 @@ from conftest import ROOT
 K = 4 * np.pi / 0.031  # rad per metre of slant range
 
 
+CSRC = os.path.join(ROOT, "sar-backprojection_amd", "csrc")
+
+
 def _limits():
-    text = open(os.path.join(ROOT, "sar-backprojection_amd", "csrc", "bp_tile.h")).read()
+    text = open(os.path.join(CSRC, "sarbp_internal.h")).read()
     get = lambda name: float(re.search(rf"{name}\s*=\s*([0-9.]+)f", text).group(1))  # noqa: E731
     return get("SPLIT_FOURTH_M"), get("SPLIT_OMAX_M")
 
@@ -75,3 +86,52 @@ def test_planted_mistakes_fail_the_bound():
     D, o, exact = _geometry(50_000, rmax, seed=4)
     assert (np.abs(_series(D, o, False) - exact) * K).max() > 20 * 4e-5  # fourth-order term dropped at 48 m
     assert (np.abs(_series(D, o, True, c_4=4.0) - exact) * K).max() > 20 * 4e-5  # wrong coefficient
+
+
+BOUND_DRIVER = r'''
+#include <cstdio>
+#include "sarbp_internal.h"
+int main() {
+  double o, r;
+  while (std::scanf("%lf %lf", &o, &r) == 2) std::printf("%.17g\n", split_truncation_m(o, r));
+}
+'''
+
+
+def _bound_tool(tmp_path):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    src = tmp_path / "bound.hip"
+    src.write_text(BOUND_DRIVER)
+    exe = tmp_path / "bound"
+    subprocess.run([hipcc, "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", f"-I{CSRC}",
+                    f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)], check=True)
+    return exe
+
+
+def _worst_truncation(r0, omax, fourth, n=200_000, seed=0):
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    D = u * (r0 / np.linalg.norm(u, axis=1))[:, None]
+    o = rng.normal(size=(n, 3))
+    o *= (omax / np.linalg.norm(o, axis=1))[:, None]
+    q = (o * o).sum(1)
+    exact = (2 * (D * o).sum(1) + q) / (np.linalg.norm(D + o, axis=1) + r0)  # |D+o| - |D|, no cancellation
+    return np.abs(_series(D, o, fourth) - exact).max()
+
+
+def test_library_truncation_bound(tmp_path):
+    r4, rmax = _limits()
+    exe = _bound_tool(tmp_path)
+    cases = [(o, r0) for r0 in (300.0, 600.0, 1000.0, 2000.0, 4200.0, 6000.0)
+             for o in (2.0, 8.0, r4, 16.0, 24.0, rmax) if r0 > 6 * o]
+    text = "".join(f"{o!r} {r0!r}\n" for o, r0 in cases)
+    out = subprocess.run([str(exe)], input=text, capture_output=True, text=True, check=True).stdout.split()
+    for (o, r0), b in zip(cases, map(float, out)):
+        worst = _worst_truncation(r0, o, fourth=o > r4)
+        print(f"omax {o} m, R0 {r0} m: bound {b:.3e} m, worst sampled {worst:.3e} m")
+        assert worst <= b <= 2.0 * worst, (o, r0)
+    # offsets not well inside the range: no bound (the kernels take FP64 terms)
+    out = subprocess.run([str(exe)], input="48 250\n10 59\n", capture_output=True, text=True, check=True).stdout
+    assert all(float(v) >= 1e29 for v in out.split())
