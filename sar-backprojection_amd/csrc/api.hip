@@ -266,6 +266,8 @@ int sarbp_dataset_destroy(sarbp_dataset *ds) {
   if (!ds) return SARBP_OK;
   if (ds->server_refs.load() > 0)
     return sarbp_fail(SARBP_ESTATE, "the dataset has server requests that are not done or failed");
+  if (ds->child_refs.load() > 0)
+    return sarbp_fail(SARBP_ESTATE, "the dataset still has scenes or batches; destroy them first");
   if (ds->stream) (void)hipStreamSynchronize(ds->stream);
   dev_free(ds->raw);
   dev_free(ds->traj);
@@ -366,6 +368,8 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
     return rc;
   }
   sc->r0min = sarbp_pixel_r0min(ds, x, y, z, n);
+  ds->child_refs.fetch_add(1);
+  sc->counted = true;
   *out = sc;
   return SARBP_OK;
 }
@@ -373,6 +377,7 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
 int sarbp_scene_destroy(sarbp_scene *sc) {
   if (!sc) return SARBP_OK;
   if (sc->ds && sc->ds->stream) (void)hipStreamSynchronize(sc->ds->stream);
+  if (sc->counted) sc->ds->child_refs.fetch_sub(1);
   dev_free(sc->x);
   dev_free(sc->y);
   dev_free(sc->z);

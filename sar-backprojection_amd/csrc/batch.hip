@@ -359,6 +359,7 @@ int sarbp_batch_create(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nre
   sarbp_batch *b = new (std::nothrow) sarbp_batch();
   if (!b) return sarbp_fail(SARBP_ENOMEM, "host allocation failed");
   b->ds = ds;
+  ds->child_refs.fetch_add(1);  // released by sarbp_batch_destroy
   b->reqs.resize((size_t)nreq);
   for (int32_t r = 0; r < nreq; ++r) {
     int rc = reqbuf_create(ds, reqs[r], 0, default_tiling(reqs[r].precision).accum, b->reqs[(size_t)r],
@@ -447,6 +448,7 @@ int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t 
 int sarbp_batch_destroy(sarbp_batch *b) {
   if (!b) return SARBP_OK;
   if (b->ds && b->ds->stream) (void)hipStreamSynchronize(b->ds->stream);
+  if (b->ds) b->ds->child_refs.fetch_sub(1);
   for (auto &q : b->reqs) reqbuf_free(q);
   b->ib.release();
   delete b;

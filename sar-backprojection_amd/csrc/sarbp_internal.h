@@ -64,6 +64,9 @@ struct sarbp_dataset {
   // requests submitted to a server against this dataset and not yet done or failed (preempted ones
   // included: they may be resumed); sarbp_dataset_destroy refuses while any is left
   std::atomic<int64_t> server_refs{0};
+  // scenes and batches created on this dataset and not yet destroyed; sarbp_dataset_destroy refuses
+  // while any is left (they use its stream and RawData)
+  std::atomic<int64_t> child_refs{0};
   BpGeom geom() const;
 };
 
@@ -89,6 +92,7 @@ struct sarbp_scene {
   int32_t *ncnt = nullptr;
   int64_t bytes = 0;
   int32_t bin_lo = 0;  // lowest contributing range bin (sarbp_scene_set_bin_min)
+  bool counted = false;  // counted in ds->child_refs
   float r0min = 0.f;   // lower bound of the pixel-to-sensor distances (sarbp_pixel_r0min)
   PixelsView view() const { return PixelsView{x, y, z, mask, index, sint, n, r0min}; }
   BpGeom geom() const;  // the dataset's geometry with this scene's range-bin floor

@@ -57,6 +57,7 @@ class Batch:
         check(_lib.load().sarbp_batch_create(ds.handle, arr, len(self.requests), ctypes.byref(h)), "sarbp_batch_create")
         self._h = h
         _lib.track(self)
+        ds._adopt(self)
 
     def run(self, tile_px: int = 0, pulse_chunk: int = 0) -> float:
         """One persistent launch over every request; returns the kernel time in ms."""

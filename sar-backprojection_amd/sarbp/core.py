@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -102,11 +103,21 @@ class Dataset:
         check(_lib.load().sarbp_dataset_bytes(self._h, ctypes.byref(v)))
         return int(v.value)
 
+    def _adopt(self, child):
+        """Scenes and batches of this dataset: closed with it (the library refuses to destroy a
+        dataset whose scenes or batches still exist)."""
+        if not hasattr(self, "_children"):
+            self._children = weakref.WeakSet()
+        self._children.add(child)
+
     def close(self):
-        """Release the device memory.  Raises SarbpError (ESTATE) while a server still has requests
-        against this dataset that are not done or failed; the dataset then stays valid."""
+        """Release the device memory, after closing the dataset's scenes and batches.  Raises
+        SarbpError (ESTATE) while a server still has requests against this dataset that are not done
+        or failed; the dataset then stays valid."""
         h = getattr(self, "_h", None)
         if h:
+            for c in list(getattr(self, "_children", ())):
+                c.close()
             check(_lib.load().sarbp_dataset_destroy(h), "sarbp_dataset_destroy")
             self._h = None
 
@@ -195,6 +206,7 @@ class Scene:
                                              self.n, self.dimx, ctypes.byref(h)), "sarbp_scene_create")
         self._h = h
         _lib.track(self)
+        ds._adopt(self)
 
     def set_tiling(self, precision="f64", kernel=1, tile_x=16, accum=0):
         """Work mapping for one precision (sarbp_scene_set_tiling)."""

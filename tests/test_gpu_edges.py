@@ -38,6 +38,8 @@ def _check(prec, got, cnt, ref, rcnt, what):
     assert np.array_equal(cnt, rcnt), (what, np.nonzero(cnt != rcnt)[0][:10])
     nz = rcnt > 0
     assert np.all(got[~nz] == 0), what
+    if not nz.any():
+        return
     rel = np.abs(got[nz] - ref[nz]) / np.abs(ref[nz])
     if prec == "f64":
         assert rel.max() <= 1e-10, (what, float(rel.max()))
@@ -93,7 +95,8 @@ def test_edges_batch_kernel(gpu, grid):
 def test_edges_serving(gpu, grid, max_segments):
     """Served in many short rounds (8-pulse chunks) or with the long apertures' pulses in segments."""
     sc, ds, ref, rcnt = grid
-    srv = Server(quantum_ms=0.01, pulse_chunk=8, max_segments=max_segments)
+    # rounds of ~3e4 pixel-pulse pairs (the server's initial rate estimate x 0.1 us): 72-pulse spans
+    srv = Server(quantum_ms=1e-4, pulse_chunk=8, max_segments=max_segments)
     hs = [srv.submit(ds, r) for r in _requests(sc)]
     for h in hs:
         assert h.wait(timeout=60)
@@ -117,8 +120,10 @@ def test_edges_bin_floor(gpu, grid, bin_min):
     rcnt = np.array([r[1] for r in refs], np.int32)
     scene = ds.scene(x, y, z, m, idx, s, dimx=sc.dimx)
     scene.set_bin_min(bin_min)
-    for prec in ("f64", "split"):
-        scene.focus(prec)
-        got, cnt = scene.result(with_counts=True)
-        _check(prec, got, cnt, ref, rcnt, ("bin floor", bin_min, prec))
-    scene.close()
+    try:
+        for prec in ("f64", "split"):
+            scene.focus(prec)
+            got, cnt = scene.result(with_counts=True)
+            _check(prec, got, cnt, ref, rcnt, ("bin floor", bin_min, prec))
+    finally:
+        scene.close()
