@@ -101,10 +101,11 @@ def test_full_split_within_tolerance_of_f64(gpu, full):
     print("split vs f64 at full size", e, flush=True)
     assert e["phase_rms_rad_rel1e-5max"] <= 1e-3
     assert e["amp_rel_rmse"] <= 1e-4
-    # per-pixel tails over the well-conditioned pixels
-    assert e["pixels_rel1e-2max"] > 100_000
-    assert e["phase_max_rad_rel1e-2max"] <= 1e-3
-    assert e["amp_rel_max_rel1e-2max"] <= 1e-3
+    # per-pixel tails over the well-conditioned pixels (~12k: the targets' responses; measured
+    # 3.8e-5 rad and 3.3e-5 at most)
+    assert e["pixels_rel1e-2max"] > 5000
+    assert e["phase_max_rad_rel1e-2max"] <= 2e-4
+    assert e["amp_rel_max_rel1e-2max"] <= 2e-4
     shape = sc.shape
     pk = metrics.target_peaks(res["split"][0].reshape(shape, order="F"), res["f64"][0].reshape(shape, order="F"),
                               sc.targets[:, :2], float(sc.x[0, 0]), float(sc.y[0, 0]),
