@@ -162,6 +162,25 @@ int sarbp_batch_download(sarbp_batch *b, int32_t req, double *out_c128, int32_t 
 int sarbp_batch_destroy(sarbp_batch *b);
 
 /*
+ * Allocation-free batch focusing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same persistent launch as sarbp_batch_run over nreq requests whose pixel arrays
+ * (reqs[k].x .. .sint) are DEVICE pointers, writing request k's means to out_c128[k] (2*n float64)
+ * and its contributor counts to ncontrib[k] (n int32), both device pointers; the results equal
+ * sarbp_batch_create + sarbp_batch_run + sarbp_batch_download bit for bit.  tile_px / pulse_chunk
+ * as sarbp_batch_run (0: 64 / 32).  The launch's work list (request descriptors, items, queue
+ * offsets and heads) lives in a fixed device array of SARBP_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends: SARBP_EINVAL when the requests need more (about 20 B per 64-pixel tile
+ * and 256 B per request: ~12 M pixels).  No call allocates host or device memory.  Calls share that
+ * array: they are serialised within a process (another call waits), and the call returns when the
+ * results are in place (it synchronises the dataset's stream).
+ */
+#define SARBP_NOALLOC_SCRATCH_BYTES (4 << 20)
+int sarbp_batch_focus_noalloc(sarbp_dataset *ds, const sarbp_request *reqs, int32_t nreq,
+                              double *const *out_c128, int32_t *const *ncontrib, int32_t tile_px,
+                              int32_t pulse_chunk);
+
+/*
  * Diagnostic: the FP64 path's arithmetic for one pixel-pulse pair (sar_focuser_puntual.py:82-121)
  * evaluated on the device for n pairs: r2 = (dx^2 + dy^2) + dz^2, r = sqrt(r2), bin =
  * rint((r - near)/dra) half-even (-1 when outside [0, nra-1)), ph = k r + beta r2 and (sin, cos)(ph),

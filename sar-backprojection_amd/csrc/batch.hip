@@ -342,12 +342,14 @@ int batch_time(sarbp_batch *b, int tile_px, int chunk, float *ms) {
   return SARBP_OK;
 }
 
-bool valid_combo(int tile_px, int chunk, int max_rows) {
+bool valid_combo(int tile_px, int chunk, int max_rows) { return sarbp_batch_mapping_valid(tile_px, chunk, max_rows); }
+}  // namespace
+
+bool sarbp_batch_mapping_valid(int tile_px, int chunk, int max_rows) {
   if (tile_px != 64 && tile_px != 128 && tile_px != 256) return false;
   if (chunk < 8 || chunk > 1024 || chunk % 8) return false;
   return sarbp_batch_lds_bytes(tile_px / 64, chunk, max_rows) <= SARBP_BATCH_LDS_MAX;
 }
-}  // namespace
 
 extern "C" {
 

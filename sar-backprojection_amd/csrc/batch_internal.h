@@ -64,6 +64,10 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 
+// tile_px in {64, 128, 256}, pulse_chunk 8..1024 (x8), and the records of max_rows tile rows fit the
+// batch kernel's LDS.
+bool sarbp_batch_mapping_valid(int tile_px, int chunk, int max_rows);
+
 struct WorkItemCost {
   BatchItem item;
   int64_t cost;

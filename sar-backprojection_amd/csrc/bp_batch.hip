@@ -149,6 +149,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   batch_loop<SARBP_BATCH_SPLIT>(reqs, items, seg, heads, C, max_rows);
 }
 
+// Clears the queue heads before a launch (a kernel rather than a memset node, so that the sequence
+// also replays correctly when a caller captures it in a graph).
+__global__ __launch_bounds__(256) void k_zero_heads(int32_t *heads) {
+  for (int i = threadIdx.x; i < (int)(SARBP_HEADS_BYTES / sizeof(int32_t)); i += blockDim.x) heads[i] = 0;
+}
+
 __global__ __launch_bounds__(256) void k_bp_combine(const BpState *__restrict__ part, int nseg, int64_t n,
                                                     double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -203,7 +209,8 @@ int sarbp_launch_batch(int device, const BatchReqDev *d_reqs, const BatchItem *d
   const size_t lds = sarbp_batch_lds_bytes(waves, chunk, max_rows);
   if (lds > SARBP_BATCH_LDS_MAX) return sarbp_fail(SARBP_EINVAL, "waves x pulse chunk x tile rows needs more than 64 KiB of LDS");
   if (nitems <= 0) return SARBP_OK;
-  hipError_t e = hipMemsetAsync(d_heads, 0, SARBP_HEADS_BYTES, st);
+  hipLaunchKernelGGL(k_zero_heads, dim3(1), dim3(256), 0, st, d_heads);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "queue reset");
   if (grid_blocks <= 0) {
     int per_cu = 0, ncu = 0;

@@ -91,6 +91,7 @@ SIGNATURES = {
     "sarbp_batch_autotune": (ctypes.c_int, [_p, _i32, _i32p, _i32p, _fp]),
     "sarbp_batch_download": (ctypes.c_int, [_p, _i32, _p, _p]),
     "sarbp_batch_destroy": (ctypes.c_int, [_p]),
+    "sarbp_batch_focus_noalloc": (ctypes.c_int, [_p, _RP, _i32, ctypes.POINTER(_p), ctypes.POINTER(_p), _i32, _i32]),
     "sarbp_split_error_estimate": (ctypes.c_int, [_p, _RP, ctypes.POINTER(_d)]),
     "sarbp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerConfig), _pp]),
     "sarbp_server_submit": (ctypes.c_int, [_p, _p, _RP, _p, _p, ctypes.POINTER(_i64)]),
