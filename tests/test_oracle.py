@@ -253,3 +253,31 @@ def test_c_oracle_reproduces_golden_fixture():
     assert np.array_equal(cnt, g["ncontrib"])
     assert np.array_equal(out, g["out"])
     assert (cnt == 0).any() and (cnt > 0).sum() > 64  # the fixture has empty and focused pixels
+
+
+def test_ind_line_ra_min_gate():
+    """process_point's ind_line_ra_min (sar_focuser_puntual.py:27, 106, 138) in both restatements:
+    the C oracle equals the NumPy restatement bit for bit, and the contributor count with a floor L
+    is the number of the pixel's in-swath bins (the oracle's own trace of bins, taken without the
+    floor) that are >= max(L, 0) -- negative floors keep every bin, since the accumulation loop drops
+    negative bins anyway; 0 reproduces the plain call."""
+    sc = synth.edge_grid_scene(end=False)
+    x, y, z, m, idx, s = sc.flat()
+    checked = 0
+    for i in range(0, x.size, 3):
+        _, bins, _ = O.trace(float(x[i]), float(y[i]), float(z[i]), int(idx[i]), int(s[i]), sc.tx, sc.ty, sc.tz,
+                             sc.Nra, sc.Naz, sc.near, sc.dra)
+        if m[i] != 1 or s[i] == -9999:
+            bins = bins[:0]
+        for lo in (-4, 0, 1, 2, 30, 62, 63):
+            v, n = O.point(x[i], y[i], z[i], m[i], idx[i], s[i], sc.tx, sc.ty, sc.tz, sc.raw, sc.near, sc.dra,
+                           sc.lam, ind_line_ra_min=lo)
+            assert n == int(np.sum(bins >= max(lo, 0))), (i, lo)
+            w, nb = B.process_point(x[i], y[i], z[i], m[i], idx[i], s[i], sc.tx, sc.ty, sc.tz, sc.raw, sc.near,
+                                    sc.dra, sc.lam, ind_line_ra_min=lo)
+            assert nb == n and w == v, (i, lo)
+            if lo == 0:
+                assert (v, n) == O.point(x[i], y[i], z[i], m[i], idx[i], s[i], sc.tx, sc.ty, sc.tz, sc.raw, sc.near,
+                                         sc.dra, sc.lam)
+            checked += n > 0
+    assert checked > 50
