@@ -8,10 +8,11 @@ namespace {
 // LDS records staged per refill: C pulses x TY rows = SPLIT_SCENE_RECS (6 KiB per wavefront).
 constexpr int SPLIT_SCENE_RECS = 128;
 
-// Six wavefronts per SIMD (at most 80 VGPRs): the hot loop fits without spills; only the rare paths
-// (FP64 terms of non-compact tiles, record staging) reload a few values from scratch.
+// Five wavefronts per SIMD (at most 96 VGPRs): the loops fit without spills (measured on the
+// 4000 x 250 x 40001 workload: 4.28 ms against 4.35 ms at six waves, where the loops spill a few
+// values, and 4.77 ms at seven).
 template <int ACCUM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_bp_split_tile(BpGeom g, PixelsView px, TileGrid tg,
                                                       double2 *__restrict__ out, int32_t *__restrict__ ncnt) {
   const int C = SPLIT_SCENE_RECS / tg.ty;  // LDS: SPLIT_SCENE_RECS records (dynamic)
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
