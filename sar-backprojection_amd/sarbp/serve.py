@@ -5,6 +5,7 @@ arrivals of sub-scene requests and reports p50/p99 latency (completion - arrival
 included) and aggregate pixel-pulse pairs per second.
 
     python -m sarbp.serve --rates 5,20,50 --duration 60
+    python -m sarbp.serve --rates 50,100,200,400,800,1600 --duration 20 --pool-per-dataset 256   # load sweep
     python -m sarbp.serve --sweep          # isolated requests by size: latency phases
 """
 from __future__ import annotations
@@ -182,6 +183,8 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
         t_call = time.perf_counter()
         pending.append((t_next, t_call, server.submit(datasets[di], req)))
         k += 1
+        if k % 16 == 0:  # a driver behind its schedule still releases finished requests
+            harvest(False)
         t_next += rng.exponential(1.0 / rate_hz)
         if log and time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
@@ -267,6 +270,7 @@ def main(argv=None):
     ap.add_argument("--quantum-ms", type=float, default=20.0)
     ap.add_argument("--max-segments", type=int, default=0, help="pulse segments per tile of small requests "
                     "(0: library default; 1: off)")
+    ap.add_argument("--pool-per-dataset", type=int, default=32, help="distinct requests per dataset in the pool")
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="latency of isolated requests by size instead")
     a = ap.parse_args(argv)
@@ -291,7 +295,8 @@ def main(argv=None):
         datasets.append(Dataset(raw, base.tx, base.ty, base.tz, base.near, base.dra, base.lam))
         scenes.append(base)
         log(f"dataset {k} resident")
-    pool = make_pool(scenes)
+    pool = make_pool(scenes, nreq_per_dataset=a.pool_per_dataset)
+    log(f"pool of {len(pool)} distinct requests")
     srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms, max_segments=a.max_segments)
     results = []
     for rate in [float(r) for r in a.rates.split(",") if r]:

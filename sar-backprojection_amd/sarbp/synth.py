@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
+import os
 
 import numpy as np
 
@@ -280,3 +281,82 @@ def edge_grid_scene(seed: int = 3, naz: int = 4001, nra: int = 64, near: float =
     f = np.asfortranarray
     return Scene(f(X), f(Y), f(Z), f(mask), f(index), f(sint), tx, ty, tz, raw, near, dra, lam, None,
                  f"edge-grid-{'end' if end else 'start'}")
+
+
+def write_long_track(data_dir: str, dim_rg: int = 1000, dim_az: int = 32000, naz: int = 403484, nra: int = 4762,
+                     ntraj: int | None = None, sint_range=(2000, 3000), ntarget: int = 20,
+                     noise_columns=(10, 11, 16000, 16001, 31990, 31991), seed: int = 5) -> dict:
+    """Write the long-track dataset of main_gpu.py (main_gpu.py:304-333: RawData 4762 x 403484,
+    trajectory files one sample longer, 403485) as the reference's input files, without holding it in
+    memory.  Track along x at 0.1 m (2 cm jitter), y = -3000 m, z = 3000 m; DEM of dim_rg x dim_az
+    pixels, 1 m across track (y) and 0.1 * naz / dim_az m along it (x), z = 20 + 10 sin(x/500) cos(y/300)
+    m; Index = the broadside pulse, Sint uniform in ``sint_range``, mask Bernoulli(0.95); lambda 0.031
+    m, near range 4000 m, 0.5 m bins.
+
+    RawData is a sparse file: zeros, plus the echoes of ``ntarget`` point targets (as in
+    :func:`_simulate_raw`, without noise) and complex Gaussian noise over the apertures of the DEM
+    columns in ``noise_columns`` (so that those columns focus to non-zero values).  Returns the names,
+    the trajectory, the scalars and the noise columns."""
+    rng = np.random.default_rng(seed)
+    os.makedirs(data_dir, exist_ok=True)
+    ntraj = naz + 1 if ntraj is None else ntraj
+    near, dra, lam = 4000.0, 0.5, 0.031
+    tx = np.arange(ntraj) * 0.1 + rng.normal(0, 0.02, ntraj)
+    ty = np.full(ntraj, -3000.0) + rng.normal(0, 0.02, ntraj)
+    tz = np.full(ntraj, 3000.0) + rng.normal(0, 0.02, ntraj)
+    from . import formats
+    names = formats.grid_names(dim_rg, dim_az, underscore=True)
+    names.update(formats.traj_names(ntraj))
+    names["raw"] = formats.raw_name(nra, naz)
+    for k, t in (("tx", tx), ("ty", ty), ("tz", tz)):
+        t.astype(np.float64).tofile(os.path.join(data_dir, names[k]))
+    for f, v in zip(formats.SCALAR_FILES, (near, dra, lam)):
+        np.array([v], np.float64).tofile(os.path.join(data_dir, f))
+    dx = 0.1 * naz / dim_az
+    ys = np.arange(dim_rg) * 1.0
+    files = {k: open(os.path.join(data_dir, names[k]), "wb") for k in ("x", "y", "z", "index", "mask", "sint")}
+    try:
+        for a0 in range(0, dim_az, 2000):  # F order: whole columns, one block at a time
+            xs = np.arange(a0, min(dim_az, a0 + 2000)) * dx
+            X, Y = np.meshgrid(xs, ys, indexing="xy")  # (dim_rg, cols): rows across track, columns along it
+            Z = 20.0 + 10.0 * np.sin(X / 500.0) * np.cos(Y / 300.0)
+            idx = np.clip(np.rint(X / 0.1), 0, naz - 1).astype(np.int32)
+            sint = rng.integers(sint_range[0], sint_range[1] + 1, X.shape).astype(np.int32)
+            mask = (rng.random(X.shape) < 0.95).astype(np.uint8)
+            for k, a in (("x", X), ("y", Y), ("z", Z), ("index", idx), ("mask", mask), ("sint", sint)):
+                np.asarray(a).flatten(order="F").tofile(files[k])
+    finally:
+        for f in files.values():
+            f.close()
+    raw_path = os.path.join(data_dir, names["raw"])
+    with open(raw_path, "wb") as f:
+        f.truncate(nra * naz * 8)  # a sparse file of zeros
+    raw = np.memmap(raw_path, dtype=np.complex64, mode="r+", shape=(nra, naz), order="F")
+    sigma = 1.0
+    for j in noise_columns:
+        c = int(round(j * dx / 0.1))
+        a0, a1 = max(0, c - sint_range[1] // 2 - 100), min(naz, c + sint_range[1] // 2 + 100)
+        for b0 in range(a0, a1, 1024):
+            b1 = min(a1, b0 + 1024)
+            n = rng.standard_normal((2, nra, b1 - b0), dtype=np.float32) * np.float32(sigma)
+            raw[:, b0:b1] = n[0] + 1j * n[1]
+    tgx = rng.uniform(0, (dim_az - 1) * dx, ntarget)
+    tgy = rng.uniform(0, ys[-1], ntarget)
+    tgz = 20.0 + 10.0 * np.sin(tgx / 500.0) * np.cos(tgy / 300.0)
+    amp = np.abs(rng.normal(0, 1, ntarget)) + 1.0
+    offs = np.arange(-SINC_HALF, SINC_HALF + 1)
+    tanb = math.tan(math.radians(BEAM_DEG))
+    for (px, py, pz, a) in zip(tgx, tgy, tgz, amp):
+        half = tanb * math.hypot(py + 3000.0, pz - 3000.0)
+        sel = np.nonzero(np.abs(tx[:naz] - px) <= half)[0]
+        R = np.sqrt((px - tx[sel]) ** 2 + (py - ty[sel]) ** 2 + (pz - tz[sel]) ** 2)
+        rb = np.rint((R - near) / dra).astype(np.int64)[:, None] + offs[None, :]
+        val = a * np.sinc((near + rb * dra - R[:, None]) / dra) * np.exp(-1j * 4.0 * math.pi * R / lam)[:, None]
+        ok = (rb >= 0) & (rb < nra)
+        # only the touched samples are written: the file stays sparse outside the echoes
+        np.add.at(raw, (rb[ok], np.broadcast_to(sel[:, None], rb.shape)[ok]), val[ok].astype(np.complex64))
+    raw.flush()
+    del raw
+    return {"names": names, "tx": tx, "ty": ty, "tz": tz, "near": near, "dra": dra, "lam": lam,
+            "noise_columns": tuple(noise_columns), "dim_rg": dim_rg, "dim_az": dim_az, "nra": nra, "naz": naz,
+            "ntraj": ntraj}
