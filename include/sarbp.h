@@ -248,6 +248,10 @@ typedef struct sarbp_server_stats {
   double gpu_busy_s;          /* summed kernel time of all rounds */
   int64_t device_bytes_in_use;
   int32_t active, queued;
+  /* the scheduling thread's own time, summed: admitting requests (allocation, uploads issued),
+   * building rounds (work lists, per-XCD queues, their upload and the launches), and rounds from
+   * their start to the end of their wait for the device */
+  double admit_s, build_s, round_wall_s;
 } sarbp_server_stats;
 
 int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
@@ -265,7 +269,9 @@ int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state);
 /* Block until done (SARBP_OK), failed (SARBP_EDEVICE) or timeout_s elapsed (SARBP_EAGAIN). */
 int sarbp_server_wait(sarbp_server *s, int64_t id, double timeout_s);
 /* Stop scheduling the request at the next round boundary, save its partial state on the host and
- * release its device memory; SARBP_ESTATE if it is already done. */
+ * release its device memory; SARBP_ESTATE if it is already done.  A request already scheduled in
+ * the round that completes it completes instead (the server queues the next round while one runs,
+ * so that is the round on the device or the one queued behind it). */
 int sarbp_server_preempt(sarbp_server *s, int64_t id);
 /* Queue a preempted request again; it continues from its saved state. */
 int sarbp_server_resume(sarbp_server *s, int64_t id);

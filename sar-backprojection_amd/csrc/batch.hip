@@ -120,8 +120,20 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
   }
 }
 
+void reqbuf_layout(int64_t n64, int state_slots, size_t off[10]) {
+  const size_t n = (size_t)n64;
+  const size_t sz[9] = {n * 8, n * 8, n * 8, n, n * 4, n * 4, n * 16, n * 4, n * sizeof(BpState) * (size_t)state_slots};
+  size_t o = 0;
+  for (int k = 0; k < 9; ++k) {
+    off[k] = o;
+    o += up(sz[k]);
+  }
+  off[9] = o;
+}
+
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
-                  hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len, float r0min) {
+                  hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len, float r0min,
+                  const void *packed) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
   if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
@@ -140,13 +152,9 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
   b.tg = make_tile_grid(b.dimx, b.dimy, tx);
   const size_t n = (size_t)r.n;
   size_t off[10];
-  size_t o = 0;
-  const size_t sz[9] = {n * 8, n * 8, n * 8, n, n * 4, n * 4, n * 16, n * 4, n * sizeof(BpState) * (size_t)state_slots};
-  for (int k = 0; k < 9; ++k) {
-    off[k] = o;
-    o += up(sz[k]);
-  }
-  off[9] = o;
+  reqbuf_layout(r.n, state_slots, off);
+  const size_t sz[6] = {n * 8, n * 8, n * 8, n, n * 4, n * 4};
+  const size_t o = off[9];
   b.bytes = (int64_t)o;
   if (o > 0) {
     b.pool = pool;
@@ -177,16 +185,17 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
   const void *src[6] = {r.x, r.y, r.z, r.mask, r.index, r.sint};
   void *dst[6] = {b.x, b.y, b.z, b.mask, b.index, b.sint};
   for (int k = 0; k < 6 && n; ++k) {
-    hipError_t e = hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, st);
+    if (packed && k > 0) break;  // one copy of the packed image
+    hipError_t e = packed ? hipMemcpyAsync(b.base, packed, off[6], hipMemcpyHostToDevice, st)
+                          : hipMemcpyAsync(dst[k], src[k], sz[k], hipMemcpyHostToDevice, st);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st);  // nothing in flight may still write the block
       reqbuf_free(b);
       return sarbp_hip_fail(e, "request upload");
     }
   }
-  if (n) {  // pixels of tiles that are never scheduled (no aperture) read as zero
-    hipError_t e = hipMemsetAsync(b.out, 0, n * 16, st);
-    if (e == hipSuccess) e = hipMemsetAsync(b.ncnt, 0, n * 4, st);
+  if (n) {  // pixels of tiles that are never scheduled (no aperture) read as zero: out and ncnt at once
+    hipError_t e = hipMemsetAsync(b.out, 0, off[8] - off[6], st);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st);
       reqbuf_free(b);
@@ -236,6 +245,7 @@ int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<
     if (need <= cap) return hipSuccess;
     if (*p) (void)hipFree(*p);
     *p = nullptr;
+    need = std::max(need, cap * 2);  // geometric growth: hipFree waits for the device
     cap = 0;
     hipError_t e = hipMalloc(p, need);
     if (e == hipSuccess) cap = need;

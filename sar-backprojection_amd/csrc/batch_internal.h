@@ -57,10 +57,15 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
 // BpState entries per pixel (0: none; serving: 1 for the state between rounds, or one per pulse
 // segment).  pool: where the device block comes from (null: hipMalloc); tile_len: precomputed by
 // request_tiles (null: computed here).
+// Byte offsets of a request's device block (each array 256-byte aligned): x, y, z, mask, index, sint
+// (the inputs, [0, off[6])), out, ncnt, state slots; off[9] is the block size.
+void reqbuf_layout(int64_t n, int state_slots, size_t off[10]);
 // r0min: the pixels' distance bound when the caller has it (negative: computed here for split
-// requests; FP64 requests do not use it).
+// requests; FP64 requests do not use it).  packed: a page-locked host image of the input arrays in
+// the block's layout (uploaded with one copy; null: the six arrays of r are copied one by one).
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b, hipStream_t st,
-                  DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr, float r0min = -1.f);
+                  DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr, float r0min = -1.f,
+                  const void *packed = nullptr);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 

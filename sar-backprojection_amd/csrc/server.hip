@@ -103,6 +103,8 @@ struct ServerReq {
   std::vector<int32_t> saved_cnt;
   std::vector<int> tile_len;   // per-tile union lengths, computed at submission (caller's thread)
   bool preempt = false;
+  bool completing = false;  // results on the host, being copied into the caller's buffers
+  bool last_round = false;  // scheduled in a round that completes it
   bool holds_ds = false;  // counted in ds->server_refs until done or failed
   double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
   int32_t rounds = 0, preemptions = 0;
@@ -129,9 +131,31 @@ double split_phase_estimate(const sarbp_dataset *ds, const SplitRows &rows) {
 
 }  // namespace
 
+// One round of the serving loop: its requests, those it completes, its work-list buffers and its
+// events.  Two alternate, so that the next round is built and queued while one runs.
+struct Round {
+  std::vector<ServerReq *> reqs, fin;
+  int64_t pp = 0;  // pixel-pulse pairs of its work items
+  double t_round = 0.0;
+  int max_rows = 1;
+  bool any_f64 = false, any_split = false;
+  bool live = false;    // launched, not yet collected
+  bool failed = false;  // a launch or copy of it failed
+  bool timed = false;   // e0 and e1 bracket its kernels
+  ItemBuffers ib;
+  hipEvent_t e0 = nullptr, e1 = nullptr, done = nullptr;  // done: after its result copies
+};
+
 struct sarbp_server {
   sarbp_server_config cfg{};
   std::thread th;
+  // completion threads: copy finished requests' results from their page-locked blocks into the
+  // callers' buffers, so that the scheduling thread goes on to the next round at once
+  static constexpr int COMPLETION_THREADS = 2;
+  std::thread th_done[COMPLETION_THREADS];
+  std::deque<ServerReq *> completing;
+  std::condition_variable cv_complete;
+  bool stop_complete = false;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::map<int64_t, std::unique_ptr<ServerReq>> reqs;
@@ -142,10 +166,13 @@ struct sarbp_server {
   bool started = false;
   int device = 0;
   int64_t slots_f64 = 4096, slots_split = 4096;  // resident wavefronts of the FP64 / split batch kernels
-  hipStream_t st = nullptr;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  ItemBuffers ib;
-  std::vector<BatchReqDev> devs;  // one round's work list (scheduling thread only, reused)
+  // three streams at the default priority: rounds' kernels (st), admissions' uploads (st_up) and
+  // results' downloads (st_dn), so that the copies of one round's requests overlap another's kernels
+  hipStream_t st = nullptr, st_up = nullptr, st_dn = nullptr;
+  hipEvent_t ev_up = nullptr;  // after the uploads of the requests admitted last
+  bool up_pending = false;     // the next round waits for ev_up
+  Round rounds[2];
+  std::vector<BatchReqDev> devs;  // the round being built: its work list (scheduling thread only, reused)
   std::vector<WorkItemCost> work;
   std::vector<BatchItem> items;
   std::vector<int32_t> seg;
@@ -157,7 +184,33 @@ struct sarbp_server {
   sarbp_server_stats stats{};
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+  void sync_all() {
+    for (hipStream_t q : {st, st_up, st_dn})
+      if (q) (void)hipStreamSynchronize(q);
+  }
+  void destroy_streams() {
+    for (hipStream_t *q : {&st, &st_up, &st_dn})
+      if (*q) {
+        (void)hipStreamDestroy(*q);
+        *q = nullptr;
+      }
+  }
+  void destroy_events() {
+    if (ev_up) (void)hipEventDestroy(ev_up);
+    ev_up = nullptr;
+    for (Round &R : rounds)
+      for (hipEvent_t *ev : {&R.e0, &R.e1, &R.done})
+        if (*ev) {
+          (void)hipEventDestroy(*ev);
+          *ev = nullptr;
+        }
+  }
   void run();
+  bool build_round(Round &R);  // under mu
+  void launch_round(Round &R);  // without mu
+  void collect_round(Round &R, std::unique_lock<std::mutex> &lk);
+  void fail_rounds(std::unique_lock<std::mutex> &lk);
+  void complete_loop();
   bool admit(ServerReq *r);
   void release(ServerReq *r) {
     reqbuf_free(r->buf);
@@ -190,16 +243,39 @@ struct sarbp_server {
   }
 };
 
+void sarbp_server::complete_loop() {
+  std::unique_lock<std::mutex> lk(mu);
+  for (;;) {
+    cv_complete.wait(lk, [this] { return stop_complete || !completing.empty(); });
+    if (completing.empty()) break;  // stop_complete and nothing left
+    ServerReq *r = completing.front();
+    completing.pop_front();
+    lk.unlock();
+    if (r->buf.n > 0 && r->host) {  // out at the block's start, ncnt after it (256-byte aligned)
+      const size_t n = (size_t)r->buf.n;
+      if (r->out) std::memcpy(r->out, r->host, n * sizeof(double2));
+      if (r->ncnt) std::memcpy(r->ncnt, r->host + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
+    }
+    lk.lock();
+    r->completing = false;
+    r->done_s = now();
+    finish(r, SARBP_REQ_DONE);
+    stats.completed++;
+    stats.pp_done += r->pp;
+    cv_done.notify_all();
+  }
+}
+
 bool sarbp_server::admit(ServerReq *r) {
   // the uploads are ordered before the round's launch on the same stream; only a saved state,
   // whose host copy is dropped below, needs the stream drained first
-  int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st, &pool, &r->tile_len,
-                         r->r0min);
+  int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st_up, &pool, &r->tile_len,
+                         r->r0min, r->host);
   if (rc) return false;
   if (r->nseg > 1 && r->buf.n) {  // segment slots of tiles shorter than the last segment stay zero
-    hipError_t e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st);
+    hipError_t e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st_up);
     if (e != hipSuccess) {
-      (void)hipStreamSynchronize(st);
+      (void)hipStreamSynchronize(st_up);
       reqbuf_free(r->buf);
       return false;
     }
@@ -209,12 +285,14 @@ bool sarbp_server::admit(ServerReq *r) {
     // the partial sums of unfinished tiles, and the results of the tiles that finished before the
     // preemption (the round loop never schedules those again)
     const size_t n = r->saved.size();
-    hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), n * sizeof(BpState), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(r->buf.out, r->saved_out.data(), n * sizeof(double2), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(r->buf.ncnt, r->saved_cnt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), n * sizeof(BpState), hipMemcpyHostToDevice, st_up);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r->buf.out, r->saved_out.data(), n * sizeof(double2), hipMemcpyHostToDevice, st_up);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r->buf.ncnt, r->saved_cnt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st_up);
+    if (e == hipSuccess) e = hipStreamSynchronize(st_up);
     if (e != hipSuccess) {
-      (void)hipStreamSynchronize(st);
+      (void)hipStreamSynchronize(st_up);
       reqbuf_free(r->buf);
       return false;
     }
@@ -226,10 +304,164 @@ bool sarbp_server::admit(ServerReq *r) {
   return true;
 }
 
+bool sarbp_server::build_round(Round &R) {
+  const int chunk = cfg.pulse_chunk;
+  R.reqs.clear();
+  R.fin.clear();
+  for (ServerReq *r : active)
+    if (!r->last_round) R.reqs.push_back(r);
+  if (R.reqs.empty()) return false;
+  // every request of the round advances every unfinished tile by `span` pulses
+  int64_t live_px = 0;
+  for (ServerReq *r : R.reqs)
+    if (r->nseg == 1) live_px += std::max<int64_t>(1, r->buf.n);
+  const double budget_pp = rate * cfg.quantum_ms * 1e-3;
+  int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)std::max<int64_t>(1, live_px)));
+  span = std::max(chunk, (span + chunk - 1) / chunk * chunk);
+  const double t_round = now();
+  R.t_round = t_round;
+  devs.clear();
+  work.clear();
+  R.max_rows = 1;
+  R.pp = 0;
+  R.any_f64 = R.any_split = false;
+  for (size_t k = 0; k < R.reqs.size(); ++k) {
+    ServerReq *r = R.reqs[k];
+    r->round_s = t_round;
+    devs.push_back(reqbuf_dev(r->ds, r->buf));
+    (r->precision == SARBP_PREC_SPLIT ? R.any_split : R.any_f64) = true;
+    if (r->precision == SARBP_PREC_SPLIT) R.max_rows = std::max(R.max_rows, r->buf.tg.ty);
+    const int64_t nt = r->buf.tg.ntiles();
+    if (r->nseg > 1) {  // every segment of every tile, this round
+      for (int64_t t = 0; t < nt; ++t) {
+        const int len = r->buf.tile_len[(size_t)t];
+        for (int j = 0; j < r->nseg && j * r->seg_len < len; ++j) {
+          const int a = j * r->seg_len;
+          const int c = std::min(len, a + r->seg_len) - a;
+          work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, a, a + r->seg_len, j}, c});
+          R.pp += (int64_t)c * 64;
+        }
+      }
+    } else {
+      const int p_end = r->off + span;
+      for (int64_t t = 0; t < nt; ++t) {
+        const int len = r->buf.tile_len[(size_t)t];
+        if (len <= r->off) continue;  // finished (or empty: its zeros are written at admission)
+        const int c = std::min(len, p_end) - r->off;
+        work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, r->off, p_end, -1}, c});
+        R.pp += (int64_t)c * 64;
+      }
+    }
+    // the request's progress once this round has run; the next round is built from it while this
+    // one is still on the device
+    r->off = r->nseg > 1 ? r->max_len : r->off + span;
+    if (r->off >= r->max_len) {
+      r->last_round = true;
+      R.fin.push_back(r);
+    }
+  }
+  build_queues(work, items, seg);
+  return true;
+}
+
+void sarbp_server::launch_round(Round &R) {
+  const int chunk = cfg.pulse_chunk;
+  hipError_t e = hipSuccess;
+  if (up_pending) {  // the requests admitted since the last round: their uploads first
+    e = hipStreamWaitEvent(st, ev_up, 0);
+    up_pending = false;
+  }
+  int rc = e == hipSuccess ? R.ib.upload(devs, items, seg, st) : SARBP_OK;
+  R.timed = false;
+  if (!rc && e == hipSuccess && !items.empty()) {
+    e = hipEventRecord(R.e0, st);
+    if (e == hipSuccess)
+      rc = sarbp_launch_batch(device, R.ib.d_reqs, R.ib.d_items, R.ib.d_seg, R.ib.d_heads, (int)items.size(),
+                              sarbp_batch_mode(R.any_f64, R.any_split), 1, chunk, R.max_rows, 0, st);
+    for (ServerReq *r : R.reqs)  // segmented requests: means from the segments' partial sums
+      if (!rc && e == hipSuccess && r->nseg > 1)
+        rc = sarbp_launch_combine(r->buf.state, r->nseg, r->buf.n, r->buf.out, r->buf.ncnt, st);
+    R.timed = !rc && e == hipSuccess;
+  }
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.e1, st);  // after the round's kernels
+  // results of the requests this round completes into their page-locked blocks (the pixel copy is
+  // no longer needed), on the download stream once the kernels are done; done follows them (and
+  // the kernels, even when no request completes)
+  if (!rc && e == hipSuccess) e = hipStreamWaitEvent(st_dn, R.e1, 0);
+  for (ServerReq *r : R.fin) {
+    if (rc || e != hipSuccess || !r->buf.n || !r->host) continue;
+    const size_t n = (size_t)r->buf.n;
+    e = hipMemcpyAsync(r->host, r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st_dn);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(r->host + (n * sizeof(double2) + 255) / 256 * 256, r->buf.ncnt, n * sizeof(int32_t),
+                         hipMemcpyDeviceToHost, st_dn);
+  }
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.done, st_dn);
+  R.failed = rc || e != hipSuccess;
+  R.live = true;
+}
+
+void sarbp_server::fail_rounds(std::unique_lock<std::mutex> &lk) {
+  lk.unlock();
+  sync_all();  // nothing may still run when the rounds' blocks return to the pool
+  lk.lock();
+  for (Round &R : rounds) {
+    if (!R.live) continue;
+    for (ServerReq *r : R.reqs) {
+      if (r->state != SARBP_REQ_RUNNING || !r->on_device) continue;  // already failed with the other round
+      release(r);
+      finish(r, SARBP_REQ_FAILED);
+      stats.failed++;
+    }
+    R.live = false;
+  }
+  active.erase(std::remove_if(active.begin(), active.end(),
+                              [](ServerReq *r) { return r->state != SARBP_REQ_RUNNING; }),
+               active.end());
+  cv_done.notify_all();
+}
+
+void sarbp_server::collect_round(Round &R, std::unique_lock<std::mutex> &lk) {
+  if (!R.live) return;
+  if (R.failed) {
+    fail_rounds(lk);
+    return;
+  }
+  lk.unlock();
+  hipError_t e = hipEventSynchronize(R.done);
+  float ms = 0.f;
+  if (e == hipSuccess && R.timed) e = hipEventElapsedTime(&ms, R.e0, R.e1);
+  lk.lock();
+  if (e != hipSuccess) {
+    fail_rounds(lk);
+    return;
+  }
+  const double t_gpu = now();
+  stats.rounds++;
+  stats.gpu_busy_s += ms * 1e-3;
+  stats.round_wall_s += t_gpu - R.t_round;
+  if (ms > 0.05f && R.pp > 0) rate = 0.7 * rate + 0.3 * (R.pp * 0.8 / (ms * 1e-3));  // lanes ~80 % active
+  for (ServerReq *r : R.reqs) {
+    r->gpu_s = t_gpu;
+    r->rounds++;
+  }
+  for (ServerReq *r : R.fin) {  // every tile wrote its means, now in the request's page-locked block
+    release(r);
+    r->completing = true;
+    completing.push_back(r);
+  }
+  R.live = false;
+  active.erase(std::remove_if(active.begin(), active.end(), [](ServerReq *r) { return r->completing; }),
+               active.end());
+  cv_complete.notify_all();
+  cv_done.notify_all();
+}
+
 void sarbp_server::run() {
   (void)hipSetDevice(device);
-  const int chunk = cfg.pulse_chunk;
   std::unique_lock<std::mutex> lk(mu);
+  int next = 0;  // the round slot to build; the other one holds the round launched last
+  bool admitted = false;
   for (;;) {
     // admission, FIFO: a request waits until a slot and its memory are free
     while (!stop && !queue.empty() && (int)active.size() < cfg.max_active) {
@@ -239,8 +471,11 @@ void sarbp_server::run() {
       bytes_in_use += r->need;
       r->state = SARBP_REQ_RUNNING;
       lk.unlock();
+      const double ta = now();
       const bool ok = admit(r);
+      const double tb = now();
       lk.lock();
+      stats.admit_s += tb - ta;
       if (!ok) {
         bytes_in_use -= r->need;
         finish(r, SARBP_REQ_FAILED);
@@ -249,13 +484,26 @@ void sarbp_server::run() {
         continue;
       }
       if (r->admit_s < 0) r->admit_s = now();
+      r->last_round = false;
       active.push_back(r);
+      admitted = true;
+    }
+    if (admitted) {  // the next round's kernels wait for these uploads (only the uploads)
+      admitted = false;
+      if (hipEventRecord(ev_up, st_up) == hipSuccess) {
+        up_pending = true;
+      } else {
+        (void)hipStreamSynchronize(st_up);
+      }
     }
     if (stop) break;
-    // requests asked to stop before any work this round
+    // requests asked to stop: their device contents are saved once no round that reads them is on
+    // the device (a request in its last round completes instead)
+    if (std::any_of(active.begin(), active.end(), [](ServerReq *r) { return r->preempt && !r->last_round; }))
+      collect_round(rounds[next ^ 1], lk);
     for (auto it = active.begin(); it != active.end();) {
       ServerReq *r = *it;
-      if (!r->preempt) {
+      if (!r->preempt || r->last_round) {
         ++it;
         continue;
       }
@@ -265,7 +513,11 @@ void sarbp_server::run() {
       r->saved_cnt.resize(n);
       lk.unlock();
       hipError_t e = hipSuccess;
-      if (n) {
+      if (up_pending) {  // a request admitted since the last round: its uploads first
+        e = hipStreamWaitEvent(st, ev_up, 0);
+        up_pending = false;
+      }
+      if (n && e == hipSuccess) {
         e = hipMemcpyAsync(r->saved.data(), r->buf.state, n * sizeof(BpState), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess)
           e = hipMemcpyAsync(r->saved_out.data(), r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st);
@@ -286,123 +538,31 @@ void sarbp_server::run() {
       it = active.erase(it);
       cv_done.notify_all();
     }
-    if (active.empty()) {
-      cv_work.wait_for(lk, std::chrono::milliseconds(20));
+    Round &R = rounds[next];
+    Round &P = rounds[next ^ 1];
+    const double tb = now();
+    if (!build_round(R)) {
+      if (P.live) {
+        collect_round(P, lk);
+      } else {
+        cv_work.wait_for(lk, std::chrono::milliseconds(20));
+      }
       continue;
     }
-    // round: every active request advances every unfinished tile by `span` pulses
-    int64_t live_px = 0;
-    for (ServerReq *r : active)
-      if (r->nseg == 1) live_px += std::max<int64_t>(1, r->buf.n);
-    const double budget_pp = rate * cfg.quantum_ms * 1e-3;
-    int span = (int)std::min<double>(1 << 30, std::ceil(budget_pp / (double)std::max<int64_t>(1, live_px)));
-    span = std::max(chunk, (span + chunk - 1) / chunk * chunk);
-    std::vector<ServerReq *> round(active);
-    const double t_round = now();
-    for (ServerReq *r : round) r->round_s = t_round;
-    devs.clear();
-    work.clear();
-    int max_rows = 1;
-    int64_t round_pp = 0;
-    bool any_f64 = false, any_split = false;
-    for (size_t k = 0; k < round.size(); ++k) {
-      ServerReq *r = round[k];
-      devs.push_back(reqbuf_dev(r->ds, r->buf));
-      (r->precision == SARBP_PREC_SPLIT ? any_split : any_f64) = true;
-      if (r->precision == SARBP_PREC_SPLIT) max_rows = std::max(max_rows, r->buf.tg.ty);
-      const int64_t nt = r->buf.tg.ntiles();
-      if (r->nseg > 1) {  // every segment of every tile, this round
-        for (int64_t t = 0; t < nt; ++t) {
-          const int len = r->buf.tile_len[(size_t)t];
-          for (int j = 0; j < r->nseg && j * r->seg_len < len; ++j) {
-            const int a = j * r->seg_len;
-            const int c = std::min(len, a + r->seg_len) - a;
-            work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, a, a + r->seg_len, j}, c});
-            round_pp += (int64_t)c * 64;
-          }
-        }
-        continue;
-      }
-      const int p_end = r->off + span;
-      for (int64_t t = 0; t < nt; ++t) {
-        const int len = r->buf.tile_len[(size_t)t];
-        if (len <= r->off) continue;  // finished (or empty: its zeros are written at admission)
-        const int c = std::min(len, p_end) - r->off;
-        work.push_back(WorkItemCost{BatchItem{(int32_t)k, (int32_t)t, r->off, p_end, -1}, c});
-        round_pp += (int64_t)c * 64;
-      }
-    }
-    // requests that complete in this round: their results are copied out in the same stream pass
-    std::vector<ServerReq *> fin;
-    for (ServerReq *r : round)
-      if (r->nseg > 1 || r->off + span >= r->max_len) fin.push_back(r);
-    build_queues(work, items, seg);
     lk.unlock();
-    float ms = 0.f;
-    int rc = ib.upload(devs, items, seg, st);
-    hipError_t e = hipSuccess;
-    if (!rc && !items.empty()) {
-      e = hipEventRecord(e0, st);
-      if (e == hipSuccess)
-        rc = sarbp_launch_batch(device, ib.d_reqs, ib.d_items, ib.d_seg, ib.d_heads, (int)items.size(),
-                                sarbp_batch_mode(any_f64, any_split), 1, chunk, max_rows, 0, st);
-      for (ServerReq *r : round)  // segmented requests: means from the segments' partial sums
-        if (!rc && e == hipSuccess && r->nseg > 1)
-          rc = sarbp_launch_combine(r->buf.state, r->nseg, r->buf.n, r->buf.out, r->buf.ncnt, st);
-      if (!rc && e == hipSuccess) e = hipEventRecord(e1, st);
-    }
-    // results of completing requests into their page-locked blocks (the pixel copy is no longer
-    // needed), then one wait for the whole round
-    for (ServerReq *r : fin) {
-      if (rc || e != hipSuccess || !r->buf.n || !r->host) continue;
-      const size_t n = (size_t)r->buf.n;
-      e = hipMemcpyAsync(r->host, r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(r->host + (n * sizeof(double2) + 255) / 256 * 256, r->buf.ncnt, n * sizeof(int32_t),
-                           hipMemcpyDeviceToHost, st);
-    }
-    if (!rc && e == hipSuccess) e = hipStreamSynchronize(st);
-    if (!rc && e == hipSuccess && !items.empty()) e = hipEventElapsedTime(&ms, e0, e1);
-    const bool failed = rc || e != hipSuccess;
-    if (failed) {
-      (void)hipStreamSynchronize(st);  // nothing of this round may still run when its blocks return to the pool
-    } else {
-      for (ServerReq *r : fin) {  // into the caller's buffers
-        if (!r->buf.n || !r->host) continue;
-        const size_t n = (size_t)r->buf.n;
-        if (r->out) std::memcpy(r->out, r->host, n * sizeof(double2));
-        if (r->ncnt) std::memcpy(r->ncnt, r->host + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
-      }
-    }
+    launch_round(R);
+    const double tl = now();
     lk.lock();
-    const double t_gpu = now();
-    for (ServerReq *r : round) r->gpu_s = t_gpu;
-    stats.rounds++;
-    stats.gpu_busy_s += ms * 1e-3;
-    if (ms > 0.05f && round_pp > 0) rate = 0.7 * rate + 0.3 * (round_pp * 0.8 / (ms * 1e-3));  // lanes ~80 % active
-    for (ServerReq *r : round) {
-      r->rounds++;
-      if (failed) {
-        release(r);
-        finish(r, SARBP_REQ_FAILED);
-        stats.failed++;
-        continue;
-      }
-      r->off = r->nseg > 1 ? r->max_len : r->off + span;
-      if (r->off >= r->max_len) {  // every tile wrote its means, copied out above
-        release(r);
-        r->done_s = now();
-        finish(r, SARBP_REQ_DONE);
-        stats.completed++;
-        stats.pp_done += r->pp;
-      }
+    stats.build_s += tl - tb;
+    if (R.failed) {
+      fail_rounds(lk);  // this round and the one before it
+      continue;
     }
-    active.erase(std::remove_if(active.begin(), active.end(),
-                                [](ServerReq *r) { return r->state != SARBP_REQ_RUNNING; }),
-                 active.end());
-    cv_done.notify_all();
+    collect_round(P, lk);  // wait for the previous round while this one is queued behind it
+    next ^= 1;
   }
-  // shutdown: fail what is left
+  // shutdown: the round on the device completes; what is left fails
+  collect_round(rounds[next ^ 1], lk);
   for (ServerReq *r : active) {
     release(r);
     finish(r, SARBP_REQ_FAILED);
@@ -453,20 +613,39 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
       s->slots_split = b;
   }
   if (e == hipSuccess) e = hipStreamCreate(&s->st);
-  if (e == hipSuccess) e = hipEventCreate(&s->e0);
-  if (e == hipSuccess) e = hipEventCreate(&s->e1);
+  if (e == hipSuccess) e = hipStreamCreate(&s->st_up);
+  if (e == hipSuccess) e = hipStreamCreate(&s->st_dn);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_up, hipEventDisableTiming);
+  for (Round &R : s->rounds) {
+    if (e == hipSuccess) e = hipEventCreate(&R.e0);
+    if (e == hipSuccess) e = hipEventCreate(&R.e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&R.done, hipEventDisableTiming);
+  }
   if (e != hipSuccess) {
     int rc = sarbp_hip_fail(e, "server stream");
-    if (s->st) (void)hipStreamDestroy(s->st);
+    s->destroy_events();
+    s->destroy_streams();
     delete s;
     return rc;
   }
+  // request blocks stay cached up to the budget or 16 GiB: freeing one waits for the device
+  s->pool.limit = (size_t)std::min<int64_t>(c.hbm_budget_bytes, (int64_t)16 << 30);
+  int started_done = 0;
   try {
+    for (; started_done < sarbp_server::COMPLETION_THREADS; ++started_done)
+      s->th_done[started_done] = std::thread([s] { s->complete_loop(); });
     s->th = std::thread([s] { s->run(); });
   } catch (...) {
-    (void)hipStreamDestroy(s->st);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop_complete = true;
+      s->cv_complete.notify_all();
+    }
+    for (int k = 0; k < started_done; ++k) s->th_done[k].join();
+    s->destroy_events();
+    s->destroy_streams();
     delete s;
-    return sarbp_fail(SARBP_ENOMEM, "could not start the scheduling thread");
+    return sarbp_fail(SARBP_ENOMEM, "could not start the server's threads");
   }
   s->started = true;
   *out = s;
@@ -534,27 +713,28 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) +
             up(nn * (int64_t)sizeof(BpState) * r->nseg);
   if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
-  // the request's own copy of the pixels, uploaded at admission: x y z (f64), index sint (i32),
-  // mask (u8), each 256-byte aligned
+  // the request's own copy of the pixels, in the device block's layout (uploaded at admission with
+  // one copy); after the upload the same block receives the results (out, then ncnt, which fit in
+  // the inputs' 33 bytes per pixel)
   const size_t n = (size_t)nn;
   if (n) {
-    const size_t o1 = (size_t)up(nn * 8), o2 = 2 * o1, o3 = 3 * o1, o4 = o3 + (size_t)up(nn * 4),
-                 o5 = o4 + (size_t)up(nn * 4);
-    r->host = s->hpool.get(o5 + n, &r->host_cap);
+    size_t off[10];
+    reqbuf_layout(nn, r->nseg, off);
+    r->host = s->hpool.get(off[6], &r->host_cap);
     if (!r->host) return sarbp_fail(SARBP_ENOMEM, "page-locked host allocation failed");
     char *h = r->host;
-    std::memcpy(h, req->x, n * 8);
-    std::memcpy(h + o1, req->y, n * 8);
-    std::memcpy(h + o2, req->z, n * 8);
-    std::memcpy(h + o3, req->index, n * 4);
-    std::memcpy(h + o4, req->sint, n * 4);
-    std::memcpy(h + o5, req->mask, n);
-    r->req.x = (const double *)h;
-    r->req.y = (const double *)(h + o1);
-    r->req.z = (const double *)(h + o2);
-    r->req.index = (const int32_t *)(h + o3);
-    r->req.sint = (const int32_t *)(h + o4);
-    r->req.mask = (const uint8_t *)(h + o5);
+    std::memcpy(h + off[0], req->x, n * 8);
+    std::memcpy(h + off[1], req->y, n * 8);
+    std::memcpy(h + off[2], req->z, n * 8);
+    std::memcpy(h + off[3], req->mask, n);
+    std::memcpy(h + off[4], req->index, n * 4);
+    std::memcpy(h + off[5], req->sint, n * 4);
+    r->req.x = (const double *)(h + off[0]);
+    r->req.y = (const double *)(h + off[1]);
+    r->req.z = (const double *)(h + off[2]);
+    r->req.mask = (const uint8_t *)(h + off[3]);
+    r->req.index = (const int32_t *)(h + off[4]);
+    r->req.sint = (const int32_t *)(h + off[5]);
   }
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) {
@@ -633,6 +813,7 @@ int sarbp_server_preempt(sarbp_server *s, int64_t id) {
     return SARBP_OK;
   }
   if (r->state != SARBP_REQ_RUNNING) return sarbp_fail(SARBP_ESTATE, "request is not queued or running");
+  if (r->completing) return sarbp_fail(SARBP_ESTATE, "request is finished; its results are being delivered");
   r->preempt = true;
   s->cv_work.notify_all();
   return SARBP_OK;
@@ -690,17 +871,23 @@ int sarbp_server_destroy(sarbp_server *s) {
     s->cv_work.notify_all();
   }
   if (s->started && s->th.joinable()) s->th.join();
-  if (s->st) (void)hipStreamSynchronize(s->st);
-  s->ib.release();
+  {  // the completion threads finish the copies still queued, then stop
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop_complete = true;
+    s->cv_complete.notify_all();
+  }
+  for (auto &t : s->th_done)
+    if (t.joinable()) t.join();
+  s->sync_all();
+  for (Round &R : s->rounds) R.ib.release();
   s->pool.release();
   for (auto &kv : s->reqs) {  // preempted requests never resumed still hold their host copy
     if (kv.second->host) (void)hipHostFree(kv.second->host);
     sarbp_server::drop_ds(kv.second.get());
   }
   s->hpool.release();
-  if (s->e0) (void)hipEventDestroy(s->e0);
-  if (s->e1) (void)hipEventDestroy(s->e1);
-  if (s->st) (void)hipStreamDestroy(s->st);
+  s->destroy_events();
+  s->destroy_streams();
   delete s;
   return SARBP_OK;
 }

@@ -60,7 +60,8 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     """sarbp_server_stats (include/sarbp.h)."""
     _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("rounds", _i64), ("pp_done", _i64),
-                ("gpu_busy_s", _d), ("device_bytes_in_use", _i64), ("active", _i32), ("queued", _i32)]
+                ("gpu_busy_s", _d), ("device_bytes_in_use", _i64), ("active", _i32), ("queued", _i32),
+                ("admit_s", _d), ("build_s", _d), ("round_wall_s", _d)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_PREEMPTED, REQ_FAILED = 0, 1, 2, 3, 4

@@ -84,6 +84,7 @@ class Server:
         self._h = h
         self._inflight = {}  # id -> Handle of every request not yet seen done or failed
         self._since_reap = 0
+        self._lock = threading.Lock()  # submit may be called from several threads
         _lib.track(self)
 
     def submit(self, ds, req: Request) -> Handle:
@@ -95,26 +96,32 @@ class Server:
         check(_lib.load().sarbp_server_submit(self._h, ds.handle, ctypes.byref(st), out.ctypes.data, cnt.ctypes.data,
                                               ctypes.byref(rid)), "sarbp_server_submit")
         hd = Handle(self, int(rid.value), ds, req, out, cnt, t)
-        self._inflight[hd.id] = hd
-        self._since_reap += 1
-        if self._since_reap >= 64:
+        with self._lock:
+            self._inflight[hd.id] = hd
+            self._since_reap += 1
+            reap = self._since_reap >= 64
+        if reap:
             self.reap()
         return hd
 
     def _settled(self, rid):
-        self._inflight.pop(rid, None)
+        with self._lock:
+            self._inflight.pop(rid, None)
 
     def reap(self) -> int:
         """Release the server's references to requests that have finished (done or failed);
         returns the number still in flight."""
-        self._since_reap = 0
+        with self._lock:
+            self._since_reap = 0
+            ids = list(self._inflight)
         L = _lib.load()
         v = ctypes.c_int32()
-        for rid in list(self._inflight):
+        for rid in ids:
             if L.sarbp_server_poll(self._h, rid, ctypes.byref(v)) == _lib.OK and v.value in (_lib.REQ_DONE,
                                                                                             _lib.REQ_FAILED):
-                self._inflight.pop(rid, None)
-        return len(self._inflight)
+                self._settled(rid)
+        with self._lock:
+            return len(self._inflight)
 
     def stats(self) -> dict:
         s = _lib.ServerStats()
@@ -142,54 +149,81 @@ def percentile(v, q):
 
 
 def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: float, seed: int = 0,
-                warmup_s: float = 2.0, log=None):
+                warmup_s: float = 2.0, log=None, submit_threads: int = 4):
     """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at the times of a seeded
     Poisson schedule of rate ``rate_hz`` for ``duration_s``; wait for all.
 
-    Latency is completion minus ARRIVAL, the request's time in the Poisson schedule: it includes
-    any delay before the driver got to submit it (a driver that falls behind the schedule shows up
-    as latency), the submission itself and the queueing.  Completion times are on the server's clock
-    (``done_s``); the driver's clock is mapped onto it by the smallest observed difference between
-    a request's ``submit_s`` (server clock, entry to submit) and the driver's time just before the
-    call.  The same statistics from the entry to submit are reported as ``submit_latency_*``.
-    Finished requests are harvested as the run goes (their result buffers released), so a long run
-    holds only the requests in flight."""
+    ``submit_threads`` driver threads take the scheduled arrivals in order, each sleeping until its
+    next arrival and submitting it (the library copies the pixels in the caller's thread, so one
+    thread caps the rate a driver can offer).  Latency is completion minus ARRIVAL, the request's
+    time in the Poisson schedule: it includes any delay before the driver got to submit it (a driver
+    that falls behind the schedule shows up as latency), the submission itself and the queueing.
+    Completion times are on the server's clock (``done_s``); the driver's clock is mapped onto it by
+    the smallest observed difference between a request's ``submit_s`` (server clock, entry to
+    submit) and the driver's time just before the call.  The same statistics from the entry to
+    submit are reported as ``submit_latency_*``.  Finished requests are harvested as the run goes
+    (their result buffers released), so a long run holds only the requests in flight."""
     rng = np.random.default_rng(seed)
-    pending = collections.deque()  # (scheduled arrival offset, driver time at the call, Handle)
-    done = []                      # (scheduled arrival offset, driver time at the call, info)
-
-    def harvest(block):
-        while pending:
-            t_arr, t_call, h = pending[0]
-            if block:
-                h.wait()
-            elif h.state() not in (_lib.REQ_DONE, _lib.REQ_FAILED):
-                break
-            done.append((t_arr, t_call, h.info()))
-            pending.popleft()
-
-    st0 = server.stats()
-    t0 = time.perf_counter()
+    arrivals = []
     t_next = 0.0
-    k = 0
-    last_log = t0
     while t_next < duration_s:
-        now = time.perf_counter() - t0
-        if t_next > now:
-            harvest(False)
-            time.sleep(min(t_next - now, 0.05))
-            continue
-        di, req = pool[k % len(pool)]
-        t_call = time.perf_counter()
-        pending.append((t_next, t_call, server.submit(datasets[di], req)))
-        k += 1
-        if k % 16 == 0:  # a driver behind its schedule still releases finished requests
-            harvest(False)
+        arrivals.append(t_next)
         t_next += rng.exponential(1.0 / rate_hz)
+    slots = [None] * len(arrivals)  # (driver time at the call, Handle) once submitted
+    done = []                       # (scheduled arrival offset, driver time at the call, info)
+    errors = []
+    lock = threading.Lock()
+    nxt = [0]
+    go = threading.Event()
+    t0 = 0.0
+
+    def submitter():
+        go.wait()
+        while not errors:
+            with lock:
+                j = nxt[0]
+                nxt[0] += 1
+            if j >= len(arrivals):
+                return
+            wait = t0 + arrivals[j] - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            di, req = pool[j % len(pool)]
+            t_call = time.perf_counter()
+            try:
+                slots[j] = (t_call, server.submit(datasets[di], req))
+            except Exception as e:  # noqa: BLE001 - re-raised in the calling thread
+                errors.append(e)
+                return
+
+    threads = [threading.Thread(target=submitter, daemon=True) for _ in range(max(1, int(submit_threads)))]
+    for t in threads:
+        t.start()
+    st0 = server.stats()
+    t0 = time.perf_counter()  # the schedule's origin: set before any submitter reads it
+    go.set()
+    # harvest in arrival order: a request is released once it and every earlier one finished
+    j, last_log = 0, t0
+    while j < len(arrivals):
+        if errors:
+            break
+        sl = slots[j]
+        if sl is not None and sl[1].state() in (_lib.REQ_DONE, _lib.REQ_FAILED):
+            done.append((arrivals[j], sl[0], sl[1].info()))
+            slots[j] = None
+            j += 1
+            continue
         if log and time.perf_counter() - last_log > 10:
             last_log = time.perf_counter()
-            log(f"rate {rate_hz}/s: {k} submitted, {len(pending)} in flight, t={now:.0f}s, {server.stats()}")
-    harvest(True)
+            log(f"rate {rate_hz}/s: {nxt[0]} submitted, {j} finished, t={last_log - t0:.0f}s, {server.stats()}")
+        if sl is not None and all(not t.is_alive() for t in threads):
+            sl[1].wait()  # every submission made: block on the oldest
+        else:
+            time.sleep(0.001)
+    for t in threads:
+        t.join()
+    if errors:
+        raise errors[0]
     t_end = time.perf_counter() - t0
     st1 = server.stats()
     # driver clock -> server clock
@@ -218,6 +252,8 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
         "pp_total": int(pp), "pp_per_s": pp / span, "wall_s": t_end, "precision_counts": prec,
         "rounds": int(rounds), "rounds_per_request": rounds / max(1, len(done)),
         "gpu_busy_frac": busy / span,
+        # the scheduling thread's time, as fractions of the span: admissions, building rounds, rounds
+        **{f"{k}_frac": (st1.get(k, 0.0) - st0.get(k, 0.0)) / span for k in ("admit_s", "build_s", "round_wall_s")},
         "server": st1,
     }
 
@@ -271,6 +307,7 @@ def main(argv=None):
     ap.add_argument("--max-segments", type=int, default=0, help="pulse segments per tile of small requests "
                     "(0: library default; 1: off)")
     ap.add_argument("--pool-per-dataset", type=int, default=32, help="distinct requests per dataset in the pool")
+    ap.add_argument("--submit-threads", type=int, default=4, help="driver threads submitting the arrivals")
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="latency of isolated requests by size instead")
     a = ap.parse_args(argv)
@@ -300,7 +337,7 @@ def main(argv=None):
     srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms, max_segments=a.max_segments)
     results = []
     for rate in [float(r) for r in a.rates.split(",") if r]:
-        res = poisson_run(srv, datasets, pool, rate, a.duration, log=log)
+        res = poisson_run(srv, datasets, pool, rate, a.duration, log=log, submit_threads=a.submit_threads)
         log(json.dumps(res))
         results.append(res)
     srv.close()

@@ -60,7 +60,8 @@ class _Server:
 def test_poisson_run_statistics():
     srv = _Server()
     pool = [(0, "s"), (0, "d")]
-    res = serve.poisson_run(srv, [None], pool, rate_hz=400.0, duration_s=0.5, seed=5, warmup_s=0.1)
+    res = serve.poisson_run(srv, [None], pool, rate_hz=400.0, duration_s=0.5, seed=5, warmup_s=0.1,
+                            submit_threads=1)
     n = res["requests"]
     assert n == srv.n and n > 100  # ~200 arrivals expected
     # the seeded schedule: the k-th arrival at the sum of the first k exponential gaps
@@ -77,7 +78,9 @@ def test_poisson_run_statistics():
     # from the scheduled arrival: the stand-in's latency plus the time from the schedule to the entry
     # to submit (the driver's own lag and the 3 ms delays); the clock offset is estimated from the
     # fastest call, so each value is exact to within that call's overhead
-    t0 = srv.calls[0] - offs[0]  # the first request is called at its scheduled time (to within microseconds)
+    # the schedule's origin: no call precedes its scheduled time and most are made on time (to within
+    # microseconds), so the earliest call relative to its arrival gives it
+    t0 = min(c - o for c, o in zip(srv.calls, offs))
     full = [srv.t[k] - (t0 + offs[k]) + (k % 10 + 1) * 1e-3 for k in keep]
     assert res["latency_p50_ms"] == pytest.approx(np.percentile(full, 50) * 1e3, abs=0.3)
     assert res["latency_p99_ms"] == pytest.approx(np.percentile(full, 99) * 1e3, abs=0.3)
@@ -89,6 +92,69 @@ def test_poisson_run_statistics():
     last_done = max(srv.t[k] + (k % 10 + 1) * 1e-3 for k in range(n))
     assert res["pp_per_s"] == pytest.approx(res["pp_total"] / (last_done - srv.t[0]), rel=1e-9)
     assert res["gpu_busy_frac"] == pytest.approx(1e-3 * n / (last_done - srv.t[0]), rel=1e-9)
+
+
+class _SlowServer:
+    """Each submission takes 2 ms in the caller's thread (the library copies the pixels there);
+    requests finish 1 ms after the entry to submit."""
+
+    def __init__(self):
+        import threading
+        self.lock = threading.Lock()
+        self.seen = {}  # arrival index -> driver time at the entry to submit
+
+    def submit(self, ds, req):
+        import time
+        t = time.perf_counter()
+        with self.lock:
+            assert req not in self.seen
+            self.seen[req] = t
+        time.sleep(2e-3)
+        return _Handle2(req, t)
+
+    def stats(self):
+        return {"rounds": 0, "gpu_busy_s": 0.0}
+
+
+class _Handle2:
+    def __init__(self, j, t):
+        self.j, self.t = j, t
+
+    def wait(self, timeout=None):
+        return True
+
+    def state(self):
+        return _lib.REQ_DONE
+
+    def info(self):
+        return {"submit_s": self.t, "done_s": self.t + 1e-3, "pp": self.j, "precision": _lib.PREC_F64}
+
+
+def test_poisson_run_threads_keep_up():
+    """At 1000 arrivals/s and 2 ms per submission one driver thread falls behind by ~0.5 s over the
+    run; eight threads submit every arrival exactly once, none before its scheduled time, and keep
+    the lag from the schedule to a few ms."""
+    import time
+    rng = np.random.default_rng(3)
+    offs, t = [], 0.0
+    while t < 0.5:
+        offs.append(t)
+        t += rng.exponential(1.0 / 1000.0)
+    pool = [(0, j) for j in range(len(offs))]  # request j is arrival j
+    srv = _SlowServer()
+    t_before = time.perf_counter()
+    res = serve.poisson_run(srv, [None], pool, rate_hz=1000.0, duration_s=0.5, seed=3, warmup_s=0.0,
+                            submit_threads=8)
+    assert res["requests"] == len(offs) and sorted(srv.seen) == list(range(len(offs)))
+    assert res["pp_total"] == sum(range(len(offs)))
+    t0 = min(srv.seen.values()) - offs[0]
+    assert t0 >= t_before - 1e-3
+    assert all(srv.seen[j] >= t0 + offs[j] - 2e-3 for j in range(len(offs)))
+    assert res["arrival_lag_p99_ms"] < 50.0
+    srv1 = _SlowServer()
+    one = serve.poisson_run(srv1, [None], pool, rate_hz=1000.0, duration_s=0.5, seed=3, warmup_s=0.0,
+                            submit_threads=1)
+    assert one["requests"] == len(offs) and one["arrival_lag_p99_ms"] > 200.0
 
 
 def test_percentile_of_empty_is_none():
