@@ -133,7 +133,7 @@ void reqbuf_layout(int64_t n64, int state_slots, size_t off[10]) {
 
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
                   hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len, float r0min,
-                  const void *packed) {
+                  const void *packed, char *host_out) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
   if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
@@ -177,8 +177,8 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
   b.mask = (uint8_t *)(p + off[3]);
   b.index = (int32_t *)(p + off[4]);
   b.sint = (int32_t *)(p + off[5]);
-  b.out = (double2 *)(p + off[6]);
-  b.ncnt = (int32_t *)(p + off[7]);
+  b.out = host_out ? (double2 *)host_out : (double2 *)(p + off[6]);
+  b.ncnt = host_out ? (int32_t *)(host_out + (off[7] - off[6])) : (int32_t *)(p + off[7]);
   b.state = state_slots > 0 ? (BpState *)(p + off[8]) : nullptr;
   b.r0min = r0min >= 0.f ? r0min
                          : r.precision == SARBP_PREC_SPLIT ? sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n) : 0.f;
@@ -194,7 +194,7 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
       return sarbp_hip_fail(e, "request upload");
     }
   }
-  if (n) {  // pixels of tiles that are never scheduled (no aperture) read as zero: out and ncnt at once
+  if (n && !host_out) {  // pixels of tiles that are never scheduled (no aperture) read as zero: out and ncnt at once
     hipError_t e = hipMemsetAsync(b.out, 0, off[8] - off[6], st);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st);

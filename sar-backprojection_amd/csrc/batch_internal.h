@@ -63,9 +63,11 @@ void reqbuf_layout(int64_t n, int state_slots, size_t off[10]);
 // r0min: the pixels' distance bound when the caller has it (negative: computed here for split
 // requests; FP64 requests do not use it).  packed: a page-locked host image of the input arrays in
 // the block's layout (uploaded with one copy; null: the six arrays of r are copied one by one).
+// host_out: page-locked host memory, already zero, that the kernels write the means (n double2)
+// and, 256-byte aligned after them, the counts (n int32) into, instead of the device block.
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b, hipStream_t st,
                   DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr, float r0min = -1.f,
-                  const void *packed = nullptr);
+                  const void *packed = nullptr, char *host_out = nullptr);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 

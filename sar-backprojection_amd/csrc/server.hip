@@ -29,10 +29,12 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// Page-locked host blocks for the requests' pixel copies, reused across requests: fresh memory
-// per request costs page faults on the submit path, and uploads from pageable memory are staged
-// through the runtime's own buffers.  Same size classes and reuse rule as DevicePool; its own lock
-// (submit runs in the callers' threads, finish in the scheduling thread).
+// Page-locked host blocks for the requests' pixel copies and results, reused across requests:
+// fresh memory per request costs page faults on the submit path, and uploads from pageable memory
+// are staged through the runtime's own buffers.  Coherent (fine-grained) memory: the kernels'
+// stores of the results reach host memory directly, visible once the round's event has completed.
+// Same size classes and reuse rule as DevicePool; its own lock (submit runs in the callers'
+// threads, finish in the scheduling and completion threads).
 struct PinnedPool {
   std::mutex mu;
   std::multimap<size_t, char *> blocks;
@@ -52,7 +54,7 @@ struct PinnedPool {
       }
     }
     char *p = nullptr;
-    if (hipHostMalloc((void **)&p, want, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&p, want, hipHostMallocCoherent) != hipSuccess) {
       (void)hipGetLastError();
       return nullptr;
     }
@@ -99,12 +101,12 @@ struct ServerReq {
   // a preempted request's device contents, saved on the host: every pixel's partial state, and the
   // means and counts already written by the tiles that finished before the preemption
   std::vector<BpState> saved;
-  std::vector<double2> saved_out;
-  std::vector<int32_t> saved_cnt;
   std::vector<int> tile_len;   // per-tile union lengths, computed at submission (caller's thread)
   bool preempt = false;
   bool completing = false;  // results on the host, being copied into the caller's buffers
   bool last_round = false;  // scheduled in a round that completes it
+  bool zero_state = false;  // segment slots to clear before the first round
+  size_t out_off = 0;       // the means and counts in the page-locked block, which the kernels write
   bool holds_ds = false;  // counted in ds->server_refs until done or failed
   double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
   int32_t rounds = 0, preemptions = 0;
@@ -166,9 +168,12 @@ struct sarbp_server {
   bool started = false;
   int device = 0;
   int64_t slots_f64 = 4096, slots_split = 4096;  // resident wavefronts of the FP64 / split batch kernels
-  // three streams at the default priority: rounds' kernels (st), admissions' uploads (st_up) and
-  // results' downloads (st_dn), so that the copies of one round's requests overlap another's kernels
-  hipStream_t st = nullptr, st_up = nullptr, st_dn = nullptr;
+  // two streams at the default priority: rounds' kernels (st) and admissions' uploads (st_up), so
+  // that the uploads of the next round's requests overlap this round's kernels.  Nothing but copies
+  // goes on st_up: a kernel there would wait for the persistent round kernel to leave the CUs, and
+  // the copies queued behind it with it.  The results need no copy: the kernels write them into the
+  // requests' page-locked blocks.
+  hipStream_t st = nullptr, st_up = nullptr;
   hipEvent_t ev_up = nullptr;  // after the uploads of the requests admitted last
   bool up_pending = false;     // the next round waits for ev_up
   Round rounds[2];
@@ -185,11 +190,11 @@ struct sarbp_server {
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
   void sync_all() {
-    for (hipStream_t q : {st, st_up, st_dn})
+    for (hipStream_t q : {st, st_up})
       if (q) (void)hipStreamSynchronize(q);
   }
   void destroy_streams() {
-    for (hipStream_t *q : {&st, &st_up, &st_dn})
+    for (hipStream_t *q : {&st, &st_up})
       if (*q) {
         (void)hipStreamDestroy(*q);
         *q = nullptr;
@@ -219,8 +224,6 @@ struct sarbp_server {
   }
   static void drop_saved(ServerReq *r) {
     std::vector<BpState>().swap(r->saved);
-    std::vector<double2>().swap(r->saved_out);
-    std::vector<int32_t>().swap(r->saved_cnt);
   }
   // A finished (done or failed) request keeps only its record: its host copy of the pixel arrays
   // and any saved state are dropped, so a long-running server holds the pixels of the requests in
@@ -251,10 +254,10 @@ void sarbp_server::complete_loop() {
     ServerReq *r = completing.front();
     completing.pop_front();
     lk.unlock();
-    if (r->buf.n > 0 && r->host) {  // out at the block's start, ncnt after it (256-byte aligned)
+    if (r->buf.n > 0 && r->host) {  // the means at out_off, the counts after them (256-byte aligned)
       const size_t n = (size_t)r->buf.n;
-      if (r->out) std::memcpy(r->out, r->host, n * sizeof(double2));
-      if (r->ncnt) std::memcpy(r->ncnt, r->host + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
+      if (r->out) std::memcpy(r->out, r->host + r->out_off, n * sizeof(double2));
+      if (r->ncnt) std::memcpy(r->ncnt, r->host + r->out_off + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
     }
     lk.lock();
     r->completing = false;
@@ -270,26 +273,17 @@ bool sarbp_server::admit(ServerReq *r) {
   // the uploads are ordered before the round's launch on the same stream; only a saved state,
   // whose host copy is dropped below, needs the stream drained first
   int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st_up, &pool, &r->tile_len,
-                         r->r0min, r->host);
+                         r->r0min, r->host, r->host ? r->host + r->out_off : nullptr);
   if (rc) return false;
-  if (r->nseg > 1 && r->buf.n) {  // segment slots of tiles shorter than the last segment stay zero
-    hipError_t e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st_up);
-    if (e != hipSuccess) {
-      (void)hipStreamSynchronize(st_up);
-      reqbuf_free(r->buf);
-      return false;
-    }
-  }
+  // segment slots of tiles shorter than the last segment stay zero: cleared on the round stream
+  // before the request's first round (a kernel on st_up would hold up the uploads behind it)
+  r->zero_state = r->nseg > 1 && r->buf.n > 0;
   if (r->off == 0) drop_saved(r);  // no progress to restore
   if (!r->saved.empty()) {
-    // the partial sums of unfinished tiles, and the results of the tiles that finished before the
-    // preemption (the round loop never schedules those again)
+    // the partial sums of unfinished tiles (the means of the tiles that finished before the
+    // preemption are in the request's page-locked block, which the kernels write)
     const size_t n = r->saved.size();
     hipError_t e = hipMemcpyAsync(r->buf.state, r->saved.data(), n * sizeof(BpState), hipMemcpyHostToDevice, st_up);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(r->buf.out, r->saved_out.data(), n * sizeof(double2), hipMemcpyHostToDevice, st_up);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(r->buf.ncnt, r->saved_cnt.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st_up);
     if (e == hipSuccess) e = hipStreamSynchronize(st_up);
     if (e != hipSuccess) {
       (void)hipStreamSynchronize(st_up);
@@ -371,6 +365,11 @@ void sarbp_server::launch_round(Round &R) {
     e = hipStreamWaitEvent(st, ev_up, 0);
     up_pending = false;
   }
+  for (ServerReq *r : R.reqs)
+    if (r->zero_state && e == hipSuccess) {
+      e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st);
+      r->zero_state = false;
+    }
   int rc = e == hipSuccess ? R.ib.upload(devs, items, seg, st) : SARBP_OK;
   R.timed = false;
   if (!rc && e == hipSuccess && !items.empty()) {
@@ -384,19 +383,7 @@ void sarbp_server::launch_round(Round &R) {
     R.timed = !rc && e == hipSuccess;
   }
   if (!rc && e == hipSuccess) e = hipEventRecord(R.e1, st);  // after the round's kernels
-  // results of the requests this round completes into their page-locked blocks (the pixel copy is
-  // no longer needed), on the download stream once the kernels are done; done follows them (and
-  // the kernels, even when no request completes)
-  if (!rc && e == hipSuccess) e = hipStreamWaitEvent(st_dn, R.e1, 0);
-  for (ServerReq *r : R.fin) {
-    if (rc || e != hipSuccess || !r->buf.n || !r->host) continue;
-    const size_t n = (size_t)r->buf.n;
-    e = hipMemcpyAsync(r->host, r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st_dn);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(r->host + (n * sizeof(double2) + 255) / 256 * 256, r->buf.ncnt, n * sizeof(int32_t),
-                         hipMemcpyDeviceToHost, st_dn);
-  }
-  if (!rc && e == hipSuccess) e = hipEventRecord(R.done, st_dn);
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.done, st);
   R.failed = rc || e != hipSuccess;
   R.live = true;
 }
@@ -509,8 +496,6 @@ void sarbp_server::run() {
       }
       const size_t n = (size_t)r->buf.n;
       r->saved.resize(n);
-      r->saved_out.resize(n);
-      r->saved_cnt.resize(n);
       lk.unlock();
       hipError_t e = hipSuccess;
       if (up_pending) {  // a request admitted since the last round: its uploads first
@@ -519,10 +504,6 @@ void sarbp_server::run() {
       }
       if (n && e == hipSuccess) {
         e = hipMemcpyAsync(r->saved.data(), r->buf.state, n * sizeof(BpState), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(r->saved_out.data(), r->buf.out, n * sizeof(double2), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess)
-          e = hipMemcpyAsync(r->saved_cnt.data(), r->buf.ncnt, n * sizeof(int32_t), hipMemcpyDeviceToHost, st);
       }
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       lk.lock();
@@ -614,7 +595,6 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
   }
   if (e == hipSuccess) e = hipStreamCreate(&s->st);
   if (e == hipSuccess) e = hipStreamCreate(&s->st_up);
-  if (e == hipSuccess) e = hipStreamCreate(&s->st_dn);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_up, hipEventDisableTiming);
   for (Round &R : s->rounds) {
     if (e == hipSuccess) e = hipEventCreate(&R.e0);
@@ -713,14 +693,15 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) +
             up(nn * (int64_t)sizeof(BpState) * r->nseg);
   if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
-  // the request's own copy of the pixels, in the device block's layout (uploaded at admission with
-  // one copy); after the upload the same block receives the results (out, then ncnt, which fit in
-  // the inputs' 33 bytes per pixel)
+  // the request's page-locked block: its pixels in the device block's layout (uploaded at
+  // admission with one copy, and again after a preemption), then the means and counts, which the
+  // kernels write over PCIe as tiles finish (zero here: pixels no tile writes read as zero)
   const size_t n = (size_t)nn;
   if (n) {
     size_t off[10];
     reqbuf_layout(nn, r->nseg, off);
-    r->host = s->hpool.get(off[6], &r->host_cap);
+    r->out_off = off[6];
+    r->host = s->hpool.get(off[8], &r->host_cap);
     if (!r->host) return sarbp_fail(SARBP_ENOMEM, "page-locked host allocation failed");
     char *h = r->host;
     std::memcpy(h + off[0], req->x, n * 8);
@@ -729,6 +710,7 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
     std::memcpy(h + off[3], req->mask, n);
     std::memcpy(h + off[4], req->index, n * 4);
     std::memcpy(h + off[5], req->sint, n * 4);
+    std::memset(h + off[6], 0, off[8] - off[6]);
     r->req.x = (const double *)(h + off[0]);
     r->req.y = (const double *)(h + off[1]);
     r->req.z = (const double *)(h + off[2]);
