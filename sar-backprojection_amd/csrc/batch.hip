@@ -240,7 +240,7 @@ void build_queues(const std::vector<WorkItemCost> &work, std::vector<BatchItem> 
 }
 
 int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
-                        const std::vector<int32_t> &seg, hipStream_t st) {
+                        const std::vector<int32_t> &seg, hipStream_t st, const std::vector<int32_t> *list) {
   auto grow = [](void **p, size_t &cap, size_t need) -> hipError_t {
     if (need <= cap) return hipSuccess;
     if (*p) (void)hipFree(*p);
@@ -255,11 +255,14 @@ int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<
   const size_t rb = std::max<size_t>(1, reqs.size()) * sizeof(BatchReqDev);
   const size_t ib = std::max<size_t>(1, items.size()) * sizeof(BatchItem);
   const size_t sb = (SARBP_BATCH_QUEUES + 1) * sizeof(int32_t);
+  const size_t nl = list ? list->size() : 0;
+  const size_t lb = std::max<size_t>(1, nl) * sizeof(int32_t);
   hipError_t e = grow((void **)&d_reqs, cap_reqs, rb);
   if (e == hipSuccess) e = grow((void **)&d_items, cap_items, ib);
+  if (e == hipSuccess && nl) e = grow((void **)&d_list, cap_list, lb);
   if (e == hipSuccess && !d_seg) e = hipMalloc((void **)&d_seg, sb);
   if (e == hipSuccess && !d_heads) e = hipMalloc((void **)&d_heads, SARBP_HEADS_BYTES);
-  const size_t stage = up(rb) + up(ib) + up(sb);
+  const size_t stage = up(rb) + up(ib) + up(sb) + up(lb);
   if (e == hipSuccess && stage > cap_stage) {  // page-locked staging: the uploads are plain DMA
     const size_t want = std::max(stage, cap_stage * 2);
     if (h_stage) (void)hipHostFree(h_stage);
@@ -272,7 +275,7 @@ int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<
     (void)hipGetLastError();
     return sarbp_fail(SARBP_ENOMEM, "work-list allocation failed");
   }
-  char *hr = h_stage, *hi = h_stage + up(rb), *hs = hi + up(ib);
+  char *hr = h_stage, *hi = h_stage + up(rb), *hs = hi + up(ib), *hl = hs + up(sb);
   if (!reqs.empty()) std::memcpy(hr, reqs.data(), reqs.size() * sizeof(BatchReqDev));
   if (!items.empty()) std::memcpy(hi, items.data(), items.size() * sizeof(BatchItem));
   std::memcpy(hs, seg.data(), seg.size() * sizeof(int32_t));
@@ -280,6 +283,10 @@ int ItemBuffers::upload(const std::vector<BatchReqDev> &reqs, const std::vector<
   if (e == hipSuccess && !items.empty())
     e = hipMemcpyAsync(d_items, hi, items.size() * sizeof(BatchItem), hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(d_seg, hs, seg.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && nl) {
+    std::memcpy(hl, list->data(), nl * sizeof(int32_t));
+    e = hipMemcpyAsync(d_list, hl, nl * sizeof(int32_t), hipMemcpyHostToDevice, st);
+  }
   if (e != hipSuccess) return sarbp_hip_fail(e, "work-list upload");
   nitems = (int)items.size();
   return SARBP_OK;
@@ -290,6 +297,7 @@ void ItemBuffers::release() {
   if (d_items) (void)hipFree(d_items);
   if (d_seg) (void)hipFree(d_seg);
   if (d_heads) (void)hipFree(d_heads);
+  if (d_list) (void)hipFree(d_list);
   if (h_stage) (void)hipHostFree(h_stage);
   *this = ItemBuffers();
 }

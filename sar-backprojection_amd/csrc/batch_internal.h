@@ -89,11 +89,12 @@ struct ItemBuffers {
   BatchReqDev *d_reqs = nullptr;
   BatchItem *d_items = nullptr;
   int32_t *d_seg = nullptr, *d_heads = nullptr;
-  size_t cap_reqs = 0, cap_items = 0;
+  int32_t *d_list = nullptr;  // indices into d_reqs (the serving loop's segmented requests)
+  size_t cap_reqs = 0, cap_items = 0, cap_list = 0;
   char *h_stage = nullptr;
   size_t cap_stage = 0;
   int nitems = 0;
   int upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
-             const std::vector<int32_t> &seg, hipStream_t st);
+             const std::vector<int32_t> &seg, hipStream_t st, const std::vector<int32_t> *list = nullptr);
   void release();
 };

@@ -172,6 +172,57 @@ __global__ __launch_bounds__(256) void k_bp_combine(const BpState *__restrict__ 
 
 }  // namespace
 
+namespace {
+
+// One request per blockIdx.y; the blocks along x stride over its slots (clear) or pixels (combine).
+constexpr int SEG_BLOCKS_X = 64;
+
+__global__ __launch_bounds__(256) void k_segments_clear(const BatchReqDev *__restrict__ reqs,
+                                                        const int32_t *__restrict__ list) {
+  const BatchReqDev &r = reqs[list[blockIdx.y]];
+  const int64_t words = r.px.n * (int64_t)r.nseg * (int64_t)(sizeof(BpState) / sizeof(int4));
+  int4 *p = reinterpret_cast<int4 *>(r.state);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_int4(0, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void k_segments_combine(const BatchReqDev *__restrict__ reqs,
+                                                          const int32_t *__restrict__ list) {
+  const BatchReqDev &r = reqs[list[blockIdx.y]];
+  const int64_t n = r.px.n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double ar = 0.0, ai = 0.0;
+    int32_t cnt = 0;
+    for (int k = 0; k < r.nseg; ++k) {  // the order of k_bp_combine: the same bits
+      const BpState s = r.state[(int64_t)k * n + i];
+      ar += s.ar;
+      ai += s.ai;
+      cnt += s.cnt;
+    }
+    store_mean(r.out, r.ncnt, i, ar, ai, cnt);
+  }
+}
+
+}  // namespace
+
+int sarbp_launch_segments_clear(const BatchReqDev *d_reqs, const int32_t *d_list, int count, hipStream_t st) {
+  if (count <= 0) return SARBP_OK;
+  if (count > 65535) return sarbp_fail(SARBP_EINVAL, "too many segmented requests for one launch");
+  hipLaunchKernelGGL(k_segments_clear, dim3(SEG_BLOCKS_X, (unsigned)count), dim3(256), 0, st, d_reqs, d_list);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return sarbp_hip_fail(e, "segment clear launch");
+  return SARBP_OK;
+}
+
+int sarbp_launch_segments_combine(const BatchReqDev *d_reqs, const int32_t *d_list, int count, hipStream_t st) {
+  if (count <= 0) return SARBP_OK;
+  if (count > 65535) return sarbp_fail(SARBP_EINVAL, "too many segmented requests for one launch");
+  hipLaunchKernelGGL(k_segments_combine, dim3(SEG_BLOCKS_X, (unsigned)count), dim3(256), 0, st, d_reqs, d_list);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return sarbp_hip_fail(e, "segment combine launch");
+  return SARBP_OK;
+}
+
 int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out, int32_t *ncnt, hipStream_t st) {
   if (n <= 0) return SARBP_OK;
   if (nseg < 1) return sarbp_fail(SARBP_EINVAL, "segment count must be positive");

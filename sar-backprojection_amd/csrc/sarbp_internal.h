@@ -227,6 +227,7 @@ struct BatchReqDev {
   TileGrid tg;
   int32_t precision;  // SARBP_PREC_*
   int32_t accum;      // split path accumulation (0 Kahan, 1 FP64 fold)
+  int32_t nseg = 1;   // pulse segments per tile (state slots per pixel when > 1)
 };
 
 // Work item: tiles [tile_lo, tile_lo + W) of one request (one per wavefront of the block), pulses
@@ -273,3 +274,8 @@ int sarbp_batch_resident_blocks(int device, int mode, int waves, int chunk, int 
 // Pixel means from the partial sums of nseg pulse segments (part[k * n + i], added in the order
 // k = 0 .. nseg-1): out[i], ncnt[i] for i < n.
 int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out, int32_t *ncnt, hipStream_t st);
+// The same for several segmented requests of one launch at once: reqs[list[k]] for k < count, each
+// with nseg slots per pixel.  sarbp_launch_segments_clear zeroes their slots beforehand (a pixel
+// contributes no slot beyond its own window, and a tile no item beyond its union).
+int sarbp_launch_segments_clear(const BatchReqDev *d_reqs, const int32_t *d_list, int count, hipStream_t st);
+int sarbp_launch_segments_combine(const BatchReqDev *d_reqs, const int32_t *d_list, int count, hipStream_t st);

@@ -39,7 +39,7 @@ struct PinnedPool {
   std::mutex mu;
   std::multimap<size_t, char *> blocks;
   size_t cached = 0;
-  size_t limit = (size_t)1 << 30;
+  size_t limit = (size_t)4 << 30;
   char *get(size_t bytes, size_t *cap) {
     const size_t want = (bytes + ((size_t)1 << 20) - 1) >> 20 << 20;
     {
@@ -105,7 +105,6 @@ struct ServerReq {
   bool preempt = false;
   bool completing = false;  // results on the host, being copied into the caller's buffers
   bool last_round = false;  // scheduled in a round that completes it
-  bool zero_state = false;  // segment slots to clear before the first round
   size_t out_off = 0;       // the means and counts in the page-locked block, which the kernels write
   bool holds_ds = false;  // counted in ds->server_refs until done or failed
   double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
@@ -178,6 +177,7 @@ struct sarbp_server {
   bool up_pending = false;     // the next round waits for ev_up
   Round rounds[2];
   std::vector<BatchReqDev> devs;  // the round being built: its work list (scheduling thread only, reused)
+  std::vector<int32_t> segmented;  // its requests with pulse segments (indices into devs)
   std::vector<WorkItemCost> work;
   std::vector<BatchItem> items;
   std::vector<int32_t> seg;
@@ -260,12 +260,19 @@ void sarbp_server::complete_loop() {
       if (r->ncnt) std::memcpy(r->ncnt, r->host + r->out_off + (n * sizeof(double2) + 255) / 256 * 256, n * sizeof(int32_t));
     }
     lk.lock();
+    char *blk = r->host;  // back to the pool after the lock is released: a block beyond the pool's
+    const size_t cap = r->host_cap;  // limit is freed, and freeing waits for the device
+    r->host = nullptr;
+    r->host_cap = 0;
     r->completing = false;
     r->done_s = now();
     finish(r, SARBP_REQ_DONE);
     stats.completed++;
     stats.pp_done += r->pp;
     cv_done.notify_all();
+    lk.unlock();
+    hpool.put(blk, cap);
+    lk.lock();
   }
 }
 
@@ -275,9 +282,6 @@ bool sarbp_server::admit(ServerReq *r) {
   int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st_up, &pool, &r->tile_len,
                          r->r0min, r->host, r->host ? r->host + r->out_off : nullptr);
   if (rc) return false;
-  // segment slots of tiles shorter than the last segment stay zero: cleared on the round stream
-  // before the request's first round (a kernel on st_up would hold up the uploads behind it)
-  r->zero_state = r->nseg > 1 && r->buf.n > 0;
   if (r->off == 0) drop_saved(r);  // no progress to restore
   if (!r->saved.empty()) {
     // the partial sums of unfinished tiles (the means of the tiles that finished before the
@@ -316,6 +320,7 @@ bool sarbp_server::build_round(Round &R) {
   R.t_round = t_round;
   devs.clear();
   work.clear();
+  segmented.clear();
   R.max_rows = 1;
   R.pp = 0;
   R.any_f64 = R.any_split = false;
@@ -323,6 +328,8 @@ bool sarbp_server::build_round(Round &R) {
     ServerReq *r = R.reqs[k];
     r->round_s = t_round;
     devs.push_back(reqbuf_dev(r->ds, r->buf));
+    devs.back().nseg = r->nseg;
+    if (r->nseg > 1 && r->buf.n > 0) segmented.push_back((int32_t)k);
     (r->precision == SARBP_PREC_SPLIT ? R.any_split : R.any_f64) = true;
     if (r->precision == SARBP_PREC_SPLIT) R.max_rows = std::max(R.max_rows, r->buf.tg.ty);
     const int64_t nt = r->buf.tg.ntiles();
@@ -365,21 +372,19 @@ void sarbp_server::launch_round(Round &R) {
     e = hipStreamWaitEvent(st, ev_up, 0);
     up_pending = false;
   }
-  for (ServerReq *r : R.reqs)
-    if (r->zero_state && e == hipSuccess) {
-      e = hipMemsetAsync(r->buf.state, 0, (size_t)r->buf.n * sizeof(BpState) * (size_t)r->nseg, st);
-      r->zero_state = false;
-    }
-  int rc = e == hipSuccess ? R.ib.upload(devs, items, seg, st) : SARBP_OK;
+  int rc = e == hipSuccess ? R.ib.upload(devs, items, seg, st, &segmented) : SARBP_OK;
   R.timed = false;
   if (!rc && e == hipSuccess && !items.empty()) {
     e = hipEventRecord(R.e0, st);
-    if (e == hipSuccess)
+    // segmented requests (each in its only round): slots cleared first, means from their partial
+    // sums after the batch kernel; one launch each for all of them (a kernel on st_up would hold
+    // up the uploads queued behind it)
+    if (e == hipSuccess) rc = sarbp_launch_segments_clear(R.ib.d_reqs, R.ib.d_list, (int)segmented.size(), st);
+    if (!rc && e == hipSuccess)
       rc = sarbp_launch_batch(device, R.ib.d_reqs, R.ib.d_items, R.ib.d_seg, R.ib.d_heads, (int)items.size(),
                               sarbp_batch_mode(R.any_f64, R.any_split), 1, chunk, R.max_rows, 0, st);
-    for (ServerReq *r : R.reqs)  // segmented requests: means from the segments' partial sums
-      if (!rc && e == hipSuccess && r->nseg > 1)
-        rc = sarbp_launch_combine(r->buf.state, r->nseg, r->buf.n, r->buf.out, r->buf.ncnt, st);
+    if (!rc && e == hipSuccess)
+      rc = sarbp_launch_segments_combine(R.ib.d_reqs, R.ib.d_list, (int)segmented.size(), st);
     R.timed = !rc && e == hipSuccess;
   }
   if (!rc && e == hipSuccess) e = hipEventRecord(R.e1, st);  // after the round's kernels
