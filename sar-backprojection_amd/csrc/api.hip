@@ -58,8 +58,11 @@ int upload_traj(sarbp_dataset *ds, const double *tx, const double *ty, const dou
     ds->tlo[k] = *std::min_element(c[k], c[k] + ds->naz);
     ds->thi[k] = *std::max_element(c[k], c[k] + ds->naz);
   }
-  std::vector<double4> t((size_t)ds->naz);
-  for (int64_t a = 0; a < ds->naz; ++a) t[(size_t)a] = make_double4(tx[a], ty[a], tz[a], 0.0);
+  std::vector<double4> t((size_t)(ds->naz + SARBP_TRAJ_PAD));
+  for (int64_t a = 0; a < ds->naz + SARBP_TRAJ_PAD; ++a) {
+    const int64_t k = std::min(a, ds->naz - 1);
+    t[(size_t)a] = make_double4(tx[k], ty[k], tz[k], 0.0);
+  }
   return h2d(ds->traj, t.data(), t.size() * sizeof(double4), ds->stream, "trajectory upload");
 }
 
@@ -101,7 +104,7 @@ int dataset_init(int64_t nra, int64_t naz, const double *tx, const double *ty, c
     return rc;
   }
   rc = dev_alloc(&ds->raw, (size_t)nra * (size_t)naz, ds->bytes, "RawData");
-  if (!rc) rc = dev_alloc(&ds->traj, (size_t)naz, ds->bytes, "trajectory");
+  if (!rc) rc = dev_alloc(&ds->traj, (size_t)(naz + SARBP_TRAJ_PAD), ds->bytes, "trajectory");
   if (!rc) rc = upload_traj(ds, tx, ty, tz);
   if (!rc) rc = dev_alloc(&ds->sctab, SARBP_SCTAB, ds->bytes, "sin/cos table");
   if (!rc) rc = upload_sctab(ds);
@@ -147,6 +150,42 @@ float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *
   const double d = std::sqrt(d2);
   if (!std::isfinite(d)) return 0.f;
   return std::nextafter((float)d, 0.f);  // rounded down
+}
+
+bool sarbp_live_range_bounds(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                             const uint8_t *mask, const int32_t *sint, int64_t n, double &rmin, double &rmax) {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+  const double *c[3] = {x, y, z};
+  bool any = false;
+  for (int64_t i = 0; i < n; ++i) {
+    if (mask[i] != 1 || sint[i] == -9999) continue;
+    any = true;
+    for (int k = 0; k < 3; ++k) {
+      const double v = c[k][i];
+      if (!std::isfinite(v)) return false;
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+    }
+  }
+  rmin = rmax = 0.0;
+  if (!any) return true;
+  double dmin = 0.0, dmax = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const double gap = std::max({0.0, lo[k] - ds->thi[k], ds->tlo[k] - hi[k]});
+    const double span = std::max(hi[k] - ds->tlo[k], ds->thi[k] - lo[k]);
+    dmin += gap * gap;
+    dmax += span * span;
+  }
+  rmin = std::sqrt(dmin) * (1.0 - 1e-12);
+  rmax = std::sqrt(dmax) * (1.0 + 1e-12);
+  return std::isfinite(rmax);
+}
+
+bool sarbp_narrow_ranges(const sarbp_dataset *ds, double r0min, double rmax) {
+  if (rmax == 0.0) return true;  // no pixel focuses
+  if (!(r0min >= 0x1p-440 && rmax <= 0x1p440)) return false;
+  const BpGeom g = ds->geom();
+  return std::fabs(g.kphase) * rmax + std::fabs(g.beta) * rmax * rmax < 0x1p24;
 }
 
 BpGeom sarbp_scene::geom() const {
@@ -368,6 +407,11 @@ int sarbp_scene_create(sarbp_dataset *ds, const double *x, const double *y, cons
     return rc;
   }
   sc->r0min = sarbp_pixel_r0min(ds, x, y, z, n);
+  {
+    double rmin = 0.0, rmax = 0.0;
+    sc->f64_narrow = sarbp_live_range_bounds(ds, x, y, z, mask, sint, n, rmin, rmax) &&
+                     sarbp_narrow_ranges(ds, rmin, rmax);
+  }
   ds->child_refs.fetch_add(1);
   sc->counted = true;
   *out = sc;

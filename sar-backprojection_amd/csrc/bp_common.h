@@ -9,9 +9,12 @@ namespace {
 //
 // sqrt_rn: the correctly rounded square root (hardware 1/sqrt estimate, then the Newton-Raphson /
 // FMA sequence whose last step rounds correctly, P. Markstein, "IA-64 and Elementary Functions",
-// 2000, ch. 8) for arguments whose square root needs no rescaling; others take __builtin_sqrt.
+// 2000, ch. 8) for arguments whose square root needs no rescaling; others take __builtin_sqrt
+// (WIDE = false: the caller guarantees x in [2^-900, 2^900] or a NaN, which comes out NaN either way).
+template <bool WIDE = true>
 __device__ __forceinline__ double sqrt_rn(double x) {
-  if (!(x >= 0x1p-900 && x <= 0x1p900)) return __builtin_sqrt(x);
+  if constexpr (WIDE)
+    if (!(x >= 0x1p-900 && x <= 0x1p900)) return __builtin_sqrt(x);
   const double y = __builtin_amdgcn_rsq(x);
   double gg = x * y, h = 0.5 * y;
   const double e = fma(-h, gg, 0.5);
@@ -29,13 +32,16 @@ __device__ __forceinline__ double sqrt_rn(double x) {
 // |r| <= pi/256; with (S, C) = (sin, cos)(j h), j = n mod 256, from tab:
 //   sin x = S cos r + C sin r,  cos x = C cos r - S sin r,
 // sin r to r^5 and cos r to r^6 (truncation below 1e-17 absolute).  tab may live in LDS or global
-// memory (same values, same arithmetic).  Larger |x| (never a slant-range phase: R < 80 km at
-// lambda = 3 cm) take the library sincos.
+// memory (same values, same arithmetic).  Larger |x| (slant ranges beyond ~80 km at lambda =
+// 3 cm) take the library sincos (WIDE = false: the caller guarantees |x| < 2^25, and the library
+// path and its registers are left out).
+template <bool WIDE = true>
 __device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double &sn, double &cs) {
-  if (!(__builtin_fabs(x) < 0x1p25)) {
-    sincos(x, &sn, &cs);
-    return;
-  }
+  if constexpr (WIDE)
+    if (!(__builtin_fabs(x) < 0x1p25)) {
+      sincos(x, &sn, &cs);
+      return;
+    }
   constexpr double INV_H = 40.743665431525205957;            // 128/pi
   constexpr double H_HI = 3.141592653589793116 / 128.0;      // pi rounded to double, / 128
   constexpr double H_LO = 1.2246467991473531772e-16 / 128.0; // (pi - that) / 128
@@ -63,6 +69,7 @@ __device__ __forceinline__ double range_bin(const BpGeom &g, double r) {
 
 // Range bin (as a double), R^2 and R of one pixel-pulse pair of process_point
 // (sar_focuser_puntual.py:82-106).  Returns false when the bin falls outside [0, nra-1).
+template <bool WIDE = true>
 __device__ __forceinline__ bool f64_bin(const BpGeom &g, double x, double y, double z, double tx, double ty,
                                         double tz, double &q, double &r2, double &r) {
 #pragma clang fp contract(off)
@@ -70,18 +77,19 @@ __device__ __forceinline__ bool f64_bin(const BpGeom &g, double x, double y, dou
   const double dy = y - ty;
   const double dz = z - tz;
   r2 = dx * dx + dy * dy + dz * dz;          // :88
-  r = sqrt_rn(r2);                           // :94
+  r = sqrt_rn<WIDE>(r2);                     // :94
   q = range_bin(g, r);                       // :100-101, half to even
   return q < (double)(g.nra - 1) && q >= (double)g.bin_lo;  // :106 (bin_lo: ind_line_ra_min, >= 0)
 }
 
 // sin and cos of the phase of sar_focuser_puntual.py:116: exp(-j ph) = (cs, -sn) (:121); tab is
 // the dataset's sin/cos table (in LDS or global memory).
+template <bool WIDE = true>
 __device__ __forceinline__ void f64_phase(const BpGeom &g, const double2 *tab, double r, double r2, double &sn,
                                           double &cs) {
 #pragma clang fp contract(off)
   const double ph = g.kphase * r + g.beta * r2;
-  sincos_tab(ph, tab, sn, cs);
+  sincos_tab<WIDE>(ph, tab, sn, cs);
 }
 
 // Copies the dataset's sin/cos table into the block's LDS (all threads of the block call this).
@@ -113,6 +121,10 @@ __device__ __forceinline__ bool f64_term(const BpGeom &g, const double2 *tab, do
   f64_acc(r2, sn, cs, d, acc_re, acc_im);
   return true;
 }
+
+// A gather byte offset beyond every column (Nra <= 2^24): the buffer's range check returns zero
+// without touching memory.
+constexpr unsigned RAW_NO_SAMPLE = 0x80000000u;
 
 // Buffer resource of the RawData column starting at col (wave-uniform: scalar registers), bins
 // [0, Nra-1): the last bin, which never contributes (sar_focuser_puntual.py:106), reads as zero too.

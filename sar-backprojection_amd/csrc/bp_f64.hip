@@ -41,7 +41,10 @@ __global__ __launch_bounds__(256) void k_bp_f64_lane(BpGeom g, PixelsView px, do
   store_mean(out, ncnt, i, acc_re, acc_im, cnt);
 }
 
-// One wavefront per tile; block = 64 threads.
+// One wavefront per tile; block = 64 threads.  WIDE = false when the scene's distance bounds keep
+// every R^2 and phase inside the direct ranges of sqrt_rn and sincos_tab (same bits, fewer
+// registers: the library fallbacks are not compiled in).
+template <bool WIDE>
 __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, TileGrid tg, double2 *__restrict__ out,
                                                     int32_t *__restrict__ ncnt) {
   __shared__ double2 tab[SARBP_SCTAB];
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, Til
   const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
   double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
-  f64_span(g, tab, t, t.wlo, t.whi, ar, ai, cnt);
+  f64_span<WIDE>(g, tab, t, t.wlo, t.whi, ar, ai, cnt);
   if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 
@@ -109,8 +112,8 @@ int sarbp_launch_f64(const sarbp_dataset *ds, const sarbp_scene *sc, const Tilin
     const TileGrid tg = make_tile_grid(sc->dimx, sc->dimy, t.tile_x);
     const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
     if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
-    hipLaunchKernelGGL(k_bp_f64_tile, dim3((unsigned)blocks), dim3(64), 0, st, sc->geom(), sc->view(), tg, sc->out,
-                       sc->ncnt);
+    hipLaunchKernelGGL(sc->f64_narrow ? k_bp_f64_tile<false> : k_bp_f64_tile<true>, dim3((unsigned)blocks), dim3(64),
+                       0, st, sc->geom(), sc->view(), tg, sc->out, sc->ncnt);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "FP64 kernel launch");

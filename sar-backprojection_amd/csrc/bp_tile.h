@@ -70,18 +70,74 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
 // FP64 terms for pulses [p0, p1) of the union (tab: the sin/cos table).  The trajectory sample is loaded one pulse ahead
 // (scalar loads), and the RawData gather is issued as soon as the bin is known, before the phase
 // and its sin/cos, which cover part of its latency.
+//
+// WIDE = false (the narrow-range variant, which has registers to spare): software-pipelined one
+// pulse deeper -- pulse a+1's range, bin and gather are issued before pulse a's phase, sin/cos and
+// accumulation, so the gather's latency is covered by a whole pulse of arithmetic.  The terms and
+// their order into the sums are the same.
+template <bool WIDE = true>
 __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0, int p1,
                                          double &ar, double &ai, int32_t &cnt) {
   if (p0 >= p1) return;
+  if constexpr (!WIDE) {
+    // two register sets, A and B, alternate: while one pulse's gather is in flight the other
+    // pulse's phase and sums are computed.  Column and trajectory pointers advance by addition
+    // (the device trajectory holds SARBP_TRAJ_PAD samples past the last pulse, so reading up to
+    // four pulses ahead needs no clamp; a column past the last pulse is only addressed with
+    // RAW_NO_SAMPLE, which the range check turns into zero without an access).
+    //
+    // every lane issues its gather (a lane without a term reads RAW_NO_SAMPLE: zero, no memory
+    // access), so the load sits on no branch and the wait before its use counts exactly the one
+    // gather issued after it
+    auto issue = [&](int a, const float2 *colp, double4 tr, bool &ok, double &r, double &r2, float2 &d) {
+      const __amdgpu_buffer_rsrc_t col = raw_column_at(colp, g.nra);  // uniform
+      double q = 0.0;
+      // the range and bin of every lane, on no branch (lanes outside their aperture are masked
+      // off in the instructions either way)
+      const bool in_bin = f64_bin<false>(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
+      ok = a < p1 && a >= t.lo && a < t.hi && in_bin;
+      const double qs = ok ? q : 0.0;  // in [bin_lo, nra-1) when ok
+      d = raw_load(col, ok ? (unsigned)(int)qs << 3 : RAW_NO_SAMPLE);
+    };
+    auto consume = [&](bool ok, double r, double r2, float2 d) {
+      if (ok) {
+        double sn, cs;
+        f64_phase<false>(g, tab, r, r2, sn, cs);
+        f64_acc(r2, sn, cs, d, ar, ai);
+        ++cnt;
+      }
+    };
+    const int64_t cstride = g.nra;
+    const float2 *colA = g.raw + (int64_t)p0 * cstride;  // pulse a's column (set A)
+    const double4 *tp = g.traj + p0;                     // tp[k]: pulse a + k
+    bool okA, okB;
+    double rA = 0.0, r2A = 0.0, rB = 0.0, r2B = 0.0;
+    float2 dA, dB;
+    issue(p0, colA, tp[0], okA, rA, r2A, dA);
+    double4 t1 = tp[1], t2 = tp[2];
+    for (int a = p0; a < p1; a += 2) {  // wave-uniform pulses a (set A) and a+1 (set B)
+      issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB);
+      const double4 t3 = tp[3];
+      consume(okA, rA, r2A, dA);
+      colA += 2 * cstride;
+      issue(a + 2, colA, t2, okA, rA, r2A, dA);
+      const double4 t4 = tp[4];
+      consume(okB, rB, r2B, dB);
+      tp += 2;
+      t1 = t3;
+      t2 = t4;
+    }
+    return;
+  }
   double4 tr = g.traj[p0];
   for (int a = p0; a < p1; ++a) {  // wave-uniform pulse
     const double4 trn = g.traj[a + 1 < p1 ? a + 1 : a];
     const __amdgpu_buffer_rsrc_t col = raw_column(g, a);
     double q, r2, r;
-    if (a >= t.lo && a < t.hi && f64_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r)) {
+    if (a >= t.lo && a < t.hi && f64_bin<WIDE>(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r)) {
       const float2 d = raw_load(col, (unsigned)(int)q << 3);
       double sn, cs;
-      f64_phase(g, tab, r, r2, sn, cs);
+      f64_phase<WIDE>(g, tab, r, r2, sn, cs);
       f64_acc(r2, sn, cs, d, ar, ai);
       ++cnt;
     }
@@ -232,7 +288,7 @@ struct SplitHalf {
   float2 d;
 };
 
-constexpr unsigned SPLIT_NO_SAMPLE = 0x80000000u;  // byte offset beyond every column (Nra <= 2^24)
+constexpr unsigned SPLIT_NO_SAMPLE = RAW_NO_SAMPLE;
 
 
 // FAST: every lane with an aperture is inside it for pulse a and every bin of the chunk is inside

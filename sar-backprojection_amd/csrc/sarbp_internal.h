@@ -94,9 +94,14 @@ struct sarbp_scene {
   int32_t bin_lo = 0;  // lowest contributing range bin (sarbp_scene_set_bin_min)
   bool counted = false;  // counted in ds->child_refs
   float r0min = 0.f;   // lower bound of the pixel-to-sensor distances (sarbp_pixel_r0min)
+  bool f64_narrow = false;  // every pair's R^2 and phase are within sarbp_narrow_ranges' bounds
   PixelsView view() const { return PixelsView{x, y, z, mask, index, sint, n, r0min}; }
   BpGeom geom() const;  // the dataset's geometry with this scene's range-bin floor
 };
+
+// Samples the device trajectory holds past pulse naz-1 (copies of the last one), so that kernels
+// may read a few pulses ahead of a loop's end without clamping the index.
+constexpr int SARBP_TRAJ_PAD = 4;
 
 // Aperture of one pixel (sar_focuser_puntual.py:57-66): J = 2*ceil(s/2) + 1 pulses starting at
 // index - ceil(|s|/2); integer forms of the reference's float ceil on an int32.
@@ -140,6 +145,15 @@ constexpr double SPLIT_TRUNC_MAX_RAD = 1e-4;
 // Lower bound of the distance from any of n pixels to any trajectory sample of ds: the distance
 // between the two bounding boxes (0 when they overlap), rounded down to float.
 float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n);
+// Bounds [rmin, rmax] of the distances between the pixels that focus (mask == 1, sint != -9999)
+// and every trajectory sample, from the two bounding boxes (rmin rounded down, rmax up); false
+// when such a pixel has a non-finite coordinate.  No such pixel: true, rmin = rmax = 0.
+bool sarbp_live_range_bounds(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                             const uint8_t *mask, const int32_t *sint, int64_t n, double &rmin, double &rmax);
+// True when every pixel-pulse pair with distance in [r0min, rmax] has R^2 inside sqrt_rn's direct
+// range [2^-900, 2^900] and a phase |k R + beta R^2| below 2^24 rad, so that the FP64 kernels may
+// drop their wide-argument paths (library sqrt and sin/cos), which such pairs never take.
+bool sarbp_narrow_ranges(const sarbp_dataset *ds, double r0min, double rmax);
 
 
 

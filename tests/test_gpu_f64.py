@@ -69,6 +69,36 @@ def test_tiny_scene_with_edge_cases(gpu, seed):
     assert (rcnt == 0).any() and (rcnt > 0).any()
 
 
+def test_far_range_scene(gpu):
+    """Slant ranges of ~83 km: phases above 2^24 rad (some above 2^25), so the scene kernel takes
+    its wide-argument variant (library sin/cos beyond 2^25); and the near-range scene again with
+    masked pixels at non-finite and far-away coordinates, which must not take the narrow variant
+    away from the pixels that focus (nor contribute)."""
+    import dataclasses
+    sc = synth.tiny_scene(seed=4)
+    far = dataclasses.replace(sc, ty=sc.ty - 80000.0)
+    r = lambda s: np.sqrt((s.x.mean() - s.tx.mean()) ** 2 + (s.y.mean() - s.ty.mean()) ** 2  # noqa: E731
+                          + (s.z.mean() - s.tz.mean()) ** 2)
+    far.near = sc.near + (r(far) - r(sc))
+    kphase = 4 * np.pi / sc.lam
+    assert kphase * r(far) > 2.0 ** 25
+    print(f"case far range R~{r(far):.0f} m", flush=True)
+    outs, _, ref, rcnt = focus_both(far, tilings=F64_MAPPINGS)
+    for (got, cnt), m in zip(outs, F64_MAPPINGS):
+        print("mapping", m, flush=True)
+        check_parity(got, cnt, ref, rcnt)
+    assert (rcnt > 0).sum() > 50
+    odd = dataclasses.replace(sc, x=sc.x.copy(order="F"), mask=sc.mask.copy(order="F"))
+    odd.mask[5, 5] = 0
+    odd.x[5, 5] = np.nan
+    odd.mask[6, 6] = 0
+    odd.x[6, 6] = 1e9
+    print("case masked pixels at nan / 1e9 m", flush=True)
+    outs, _, ref, rcnt = focus_both(odd, tilings=F64_MAPPINGS)
+    for (got, cnt), m in zip(outs, F64_MAPPINGS):
+        check_parity(got, cnt, ref, rcnt)
+
+
 def test_plain_pixel_list_and_odd_grid(gpu):
     # a plain list (dimx = 0) and a grid whose sides are not multiples of the tile
     sc = synth.tiny_scene(seed=9, nx=37, ny=5)
