@@ -291,18 +291,17 @@ struct SplitHalf {
 constexpr unsigned SPLIT_NO_SAMPLE = RAW_NO_SAMPLE;
 
 
-// FAST: every lane with an aperture is inside it for pulse a and every bin of the chunk is inside
-// the swath (the caller checks both), so no per-lane aperture or bin-range test is needed.
-template <bool FOURTH, bool FAST>
-__device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                            const PulseRec *rk, int a, __amdgpu_buffer_rsrc_t col,
-                                            float inv_dra, unsigned nbin_bytes, SplitHalf &h) {
+// Range offset, bin and byte offset of one lane's pixel for the pulse of record rk (no
+// aperture or swath test); fr = |qb - rint(qb)|: above s.guard the bin lies within the FP32 error
+// bound of a rounding tie and must be decided exactly (exact_off).
+template <bool FOURTH>
+__device__ __forceinline__ void split_bin(const SplitTile &s, const PulseRec *rk, float inv_dra, SplitHalf &h,
+                                          unsigned &off, float &fr) {
   const float4 ra = rk->a, rb = rk->b, rc = rk->c;
   h.t0 = rb.w;
   h.c1 = rc.x;
   h.r0sq = rc.y;
   h.two_r0 = rc.z;
-  const bool act = FAST || (a >= t.lo && a < t.hi);  // FAST: lanes without an aperture are poisoned below
   // computed on every lane (lanes outside their aperture have zero offsets: finite, unused)
   const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
   const float t2 = fmaf(-av, av, s.qo);
@@ -314,21 +313,32 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
   const float qb = fmaf(dR, inv_dra, rb.y);
   const float rq = __builtin_rintf(qb);
   // byte offset of bin B0 + rq (|bin| < 2^26: no wrap-around)
-  unsigned off = ((unsigned)(int)rq << 3) + (unsigned)__float_as_int(rb.z);
-  if (act && fabsf(qb - rq) > s.guard) {  // near a rounding tie: decide exactly (rare)
-    const double4 tr = g.traj[a];
-    const int eb = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);  // -1: outside [bin_lo, nra-1)
-    off = (unsigned)(eb - g.bin_lo) << 3;  // -1 - bin_lo -> far beyond the column
-  }
-  if (FAST) {
-    h.bad = false;
-    off |= s.poison;  // beyond the column for a lane without an aperture
-  } else {
-    const bool ok = off < nbin_bytes;  // bin_lo <= bin < Nra-1
-    h.bad = act && !ok;
-    off = act && ok ? off : SPLIT_NO_SAMPLE;
-  }
-  h.d = raw_load(col, off);
+  off = ((unsigned)(int)rq << 3) + (unsigned)__float_as_int(rb.z);
+  fr = fabsf(qb - rq);
+}
+
+// The byte offset of pulse a's bin decided in FP64 (the path of f64_bin), from bin_lo; -1 - bin_lo
+// (far beyond the column) when the bin is outside [bin_lo, nra-1).
+__device__ __forceinline__ unsigned exact_off(const BpGeom &g, const TileLane &t, int a) {
+  const double4 tr = g.traj[a];
+  const int eb = exact_bin(g, t.x, t.y, t.z, tr.x, tr.y, tr.z);  // -1: outside [bin_lo, nra-1)
+  return (unsigned)(eb - g.bin_lo) << 3;
+}
+
+// One pulse of the general form: aperture and swath tests per lane, near-tie bins decided
+// exactly, then the gather.
+template <bool FOURTH>
+__device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, const SplitTile &s,
+                                            const PulseRec *rk, int a, __amdgpu_buffer_rsrc_t col,
+                                            float inv_dra, unsigned nbin_bytes, SplitHalf &h) {
+  unsigned off;
+  float fr;
+  split_bin<FOURTH>(s, rk, inv_dra, h, off, fr);
+  const bool act = a >= t.lo && a < t.hi;
+  if (act && fr > s.guard) off = exact_off(g, t, a);  // near a rounding tie (rare)
+  const bool ok = off < nbin_bytes;  // bin_lo <= bin < Nra-1
+  h.bad = act && !ok;
+  h.d = raw_load(col, act && ok ? off : SPLIT_NO_SAMPLE);
 }
 
 // Pulses are processed four at a time: the four gathers are in flight before any term is formed;
@@ -346,10 +356,34 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
                                c2r = raw_split_column(col + 2 * nra, nbin_bytes),
                                c3r = raw_split_column(col + 3 * nra, nbin_bytes);
   SplitHalf X, Y, Z, W;
-  split_first<FOURTH, FAST>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
-  split_first<FOURTH, FAST>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
-  split_first<FOURTH, FAST>(g, t, s, rk + 2 * rstep, a + 2, c2r, inv_dra, nbin_bytes, Z);
-  split_first<FOURTH, FAST>(g, t, s, rk + 3 * rstep, a + 3, c3r, inv_dra, nbin_bytes, W);
+  if (FAST) {
+    // every lane with an aperture is inside it for the four pulses and every bin is inside the
+    // swath (the caller checks both): no per-lane tests, and the rare near-tie bins of the four
+    // pulses are decided on one branch
+    unsigned oX, oY, oZ, oW;
+    float fX, fY, fZ, fW;
+    split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
+    split_bin<FOURTH>(s, rk + rstep, inv_dra, Y, oY, fY);
+    split_bin<FOURTH>(s, rk + 2 * rstep, inv_dra, Z, oZ, fZ);
+    split_bin<FOURTH>(s, rk + 3 * rstep, inv_dra, W, oW, fW);
+    if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
+      __asm__ volatile("");  // keeps the branch: near-tie bins are rare
+      if (fX > s.guard) oX = exact_off(g, t, a);
+      if (fY > s.guard) oY = exact_off(g, t, a + 1);
+      if (fZ > s.guard) oZ = exact_off(g, t, a + 2);
+      if (fW > s.guard) oW = exact_off(g, t, a + 3);
+    }
+    X.bad = Y.bad = Z.bad = W.bad = false;
+    X.d = raw_load(c0r, oX | s.poison);  // beyond the column for a lane without an aperture
+    Y.d = raw_load(c1r, oY | s.poison);
+    Z.d = raw_load(c2r, oZ | s.poison);
+    W.d = raw_load(c3r, oW | s.poison);
+  } else {
+    split_first<FOURTH>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
+    split_first<FOURTH>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
+    split_first<FOURTH>(g, t, s, rk + 2 * rstep, a + 2, c2r, inv_dra, nbin_bytes, Z);
+    split_first<FOURTH>(g, t, s, rk + 3 * rstep, a + 3, c3r, inv_dra, nbin_bytes, W);
+  }
   const float2 tx = split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
   const float2 ty = split_term(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
   const float2 tz = split_term(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
@@ -416,7 +450,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     }
     for (; k < cn; ++k, rk += rstep, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
-      split_first<FOURTH, false>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
+      split_first<FOURTH>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
       split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
       cnt -= X.bad ? 1 : 0;
     }
