@@ -57,6 +57,36 @@ class _NpyStore:
         del self.slc
 
 
+def preview_window(dim_az, preview_size=1000, processed_az=None):
+    """Columns [start, end) of the preview (main_gpu.py:641-659): the whole image when it has at
+    most ``preview_size`` columns; else ``preview_size`` columns centred on the processed part
+    (``processed_azimuth`` columns done) when more than ``preview_size`` are processed, the
+    processed columns when fewer, and the image centre when the attribute is absent."""
+    if dim_az <= preview_size:
+        return 0, dim_az
+    if processed_az is not None:
+        processed_az = int(processed_az)
+        if processed_az > preview_size:
+            start = max(0, min(processed_az, dim_az) // 2 - preview_size // 2)
+            return start, min(dim_az, start + preview_size)
+        return 0, min(dim_az, processed_az)
+    start = max(0, dim_az // 2 - preview_size // 2)
+    return start, min(dim_az, start + preview_size)
+
+
+def preview_scale(amplitude):
+    """Display range of the linear preview (main_gpu.py:677-690): vmax = min(15 x median, max)
+    of the amplitudes, vmin = the 1st percentile of the non-zero amplitudes (0 when none)."""
+    a = np.asarray(amplitude, np.float64).ravel()
+    a = a[np.isfinite(a)]
+    if a.size == 0:
+        return 0.0, 0.0
+    vmax = min(15.0 * float(np.median(a)), float(a.max()))
+    nz = a[a > 0]
+    vmin = float(np.percentile(nz, 1)) if nz.size else 0.0
+    return vmin, vmax
+
+
 def _store_path(output_file):
     root, ext = os.path.splitext(output_file)
     return output_file if (h5py is not None or ext == ".npy") else root + ".npy"
@@ -241,8 +271,20 @@ class SARSuperOptimizedFocuser:
                                                         "dtype": "complex128"})
         return written
 
+    def _read_attrs(self):
+        path = _store_path(self.output_file)
+        if h5py is not None and not path.endswith(".npy"):
+            with h5py.File(path, "r") as f:
+                return dict(f.attrs)
+        ap = path + ".attrs.json"
+        return json.load(open(ap)) if os.path.exists(ap) else {}
+
     def generate_preview(self, preview_size=1000):
-        """Amplitude PNG of (a window of) the focused image (main_gpu.py:635-743)."""
+        """Amplitude previews of the focused image (main_gpu.py:635-743): a gray linear-scale
+        window (the columns of :func:`preview_window`, the amplitude range of
+        :func:`preview_scale`), the same window on a log10 scale, and, for images up to 10000
+        columns, the whole image at the window's scale.  Returns the linear preview's path (None
+        without matplotlib)."""
         try:
             import matplotlib
             matplotlib.use("Agg")
@@ -251,17 +293,52 @@ class SARSuperOptimizedFocuser:
             print("Matplotlib not installed. Cannot generate preview.")
             return None
         slc = self._read_slc()
-        a0 = max(0, slc.shape[1] // 2 - preview_size // 2)
-        amp = np.nan_to_num(np.abs(slc[:, a0:a0 + preview_size]))
-        med = float(np.median(amp)) if amp.size else 0.0
-        fig = plt.figure(figsize=(12, 10))
-        plt.imshow(amp, cmap="gray", vmin=0, vmax=min(15 * med, float(amp.max()) if amp.size else 1.0) or 1.0,
-                   aspect="auto")
-        plt.colorbar(label="Amplitude")
-        fn = os.path.join(self.checkpoint_dir, "SlcCuPy_SuperOptimized_Preview.png")
-        plt.savefig(fn, dpi=100)
-        plt.close(fig)
-        return fn
+        dim_rg, dim_az = slc.shape
+        a0, a1 = preview_window(dim_az, preview_size, self._read_attrs().get("processed_azimuth"))
+        if dim_az > preview_size:
+            print(f"Generating preview for columns {a0}-{a1}...")
+        amp = np.abs(slc[:, a0:a1])
+        bad = int(np.sum(~np.isfinite(amp)))
+        if bad:
+            print(f"Warning: {bad} invalid (NaN/Inf) values detected in preview region")
+            amp = np.nan_to_num(amp, nan=0.0, posinf=0.0, neginf=0.0)
+        if amp.size == 0:
+            print("Error: No valid data for preview generation")
+            return None
+        vmin, vmax = preview_scale(amp)
+        print(f"Amplitude statistics: mean={amp.mean():.2e}, median={np.median(amp):.2e}, max={amp.max():.2e}")
+        out = []
+        for kind, data, kw, cbar, title, name in (
+                ("linear", amp, dict(cmap="gray", vmin=vmin, vmax=vmax), "Amplitude",
+                 "SAR Image Amplitude (Preview)", "SlcCuPy_SuperOptimized_Preview.png"),
+                ("log", np.log10(amp + 1e-10), dict(cmap="viridis"), "Log10(Amplitude)",
+                 "SAR Image Amplitude (Log Scale)", "SlcCuPy_SuperOptimized_LogScale_Preview.png")):
+            fig = plt.figure(figsize=(12, 10))
+            plt.imshow(data, aspect="auto", **kw)
+            plt.colorbar(label=cbar)
+            plt.title(f"{title}\nColumns {a0}-{a1} of {dim_az}")
+            plt.xlabel("Azimuth Samples")
+            plt.ylabel("Range Samples")
+            plt.tight_layout()
+            fn = os.path.join(self.checkpoint_dir, name)
+            plt.savefig(fn, dpi=150)
+            plt.close(fig)
+            print(f"{'Preview' if kind == 'linear' else 'Log-scale preview'} saved to {fn}")
+            out.append(fn)
+        if dim_az <= 10000:
+            full = np.nan_to_num(np.abs(slc), nan=0.0, posinf=0.0, neginf=0.0)
+            fig = plt.figure(figsize=(16, 12))
+            plt.imshow(full, cmap="gray", vmin=vmin, vmax=vmax, aspect="auto")
+            plt.colorbar(label="Amplitude")
+            plt.title("SAR Image Amplitude (Full Resolution)")
+            plt.xlabel("Azimuth Samples")
+            plt.ylabel("Range Samples")
+            plt.tight_layout()
+            fn = os.path.join(self.checkpoint_dir, "SlcCuPy_SuperOptimized_Full.png")
+            plt.savefig(fn, dpi=150)
+            plt.close(fig)
+            print(f"Full resolution image saved to {fn}")
+        return out[0]
 
     def close(self):
         if self._dataset is not None:
