@@ -28,6 +28,25 @@ from sarbp.metrics import split_error_report  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_PP = 8.0     # one complex64 RawData sample gathered per pixel-pulse pair
+# wave64 VALU instructions MI355X issues per second: 256 CUs x 4 SIMDs, one per 4 cycles, 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
+PROFILE_JSON = "profiles/traffic_full-4000x250x40001.json"
+KERNEL_OF = {"split": "k_bp_split_tile<0>", "f64": "k_bp_f64_tile<false>"}
+
+
+def default_maps(args):
+    return args.split_map == "16,0" and args.f64_map == "1,8"
+
+
+def _load_profile(pp):
+    """Per-launch counters of the default kernels on this workload, from the committed rocprofv3
+    summary (None when absent or for another workload)."""
+    fn = os.path.join(ROOT, PROFILE_JSON)
+    if not os.path.exists(fn):
+        return None
+    with open(fn) as f:
+        tj = json.load(f)
+    return tj if tj.get("pp_per_launch") == pp else None
 
 
 def log(msg):
@@ -135,11 +154,12 @@ def target_peaks(sc, img_split, img_f64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--strip", type=int, default=1024, help="pixel columns of the all-cores CPU baseline strip")
-    ap.add_argument("--serial-strip", type=int, default=64, help="pixel columns of the 1-thread CPU strip")
+    ap.add_argument("--strip", type=int, default=4000, help="pixel columns of the all-cores CPU baseline strip "
+                    "(4000: the whole scene, ~3.5 s on 16 cores)")
+    ap.add_argument("--serial-strip", type=int, default=512, help="pixel columns of the 1-thread CPU strip (~7 s)")
     ap.add_argument("--precisions", default="f64,split")
     ap.add_argument("--small", action="store_true", help="scaled-down scene (quick check, not the metric)")
     ap.add_argument("--f64-map", default="1,8", help="FP64 work mapping kernel,tile_x")
@@ -148,6 +168,8 @@ def main():
     ap.add_argument("--batch-requests", type=int, default=64, help="sub-scene requests of the batch leg (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=10.0, help="Poisson serving leg duration (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate of the serving leg")
+    ap.add_argument("--serve-load-rate", type=float, default=1200.0,
+                    help="arrival rate of the sustained-load serving leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (pixel sharding across GPUs "
@@ -204,13 +226,21 @@ def main():
     extra = {"pp_per_step": pp, "pixels": int(scene.n), "f64_map": args.f64_map, "split_map": args.split_map}
     if sweep:
         extra["sweep_pp_per_s"] = sweep
+    prof = _load_profile(pp) if not args.small and not args.sweep else None
     for p, r in res.items():
         extra[f"{p}_pp_per_s"] = r["pp_per_s"]
         extra[f"{p}_kernel_ms"] = r["kernel_s"] * 1e3
-        # the same gather-bandwidth roofline for each path, and the FP64-VALU one (26 flop/pp
-        # against 78.6 TFLOP/s) for reference
+        # the same gather-bandwidth roofline for each path
         extra[f"{p}_roofline_frac"] = BYTES_PER_PP * pp / r["kernel_s"] / 1e9 / HBM_PEAK_GBS
-        extra[f"{p}_fp64_valu_frac"] = 26.0 * pp / r["kernel_s"] / 78.6e12
+        # the VALU-issue ceiling: the kernel's wave-level VALU instructions per launch (rocprofv3
+        # SQ_INSTS_VALU, transcendentals at four issue cycles) over what 256 CUs x 4 SIMDs issue
+        # in this launch's time, one wave64 instruction per 4 cycles at 2.4 GHz
+        kp = (prof or {}).get("kernels", {}).get(KERNEL_OF[p])
+        if kp and default_maps(args):
+            c = kp["counters"]
+            issue = c["SQ_INSTS_VALU"] + 3.0 * (c.get("SQ_INSTS_VALU_TRANS_F32", 0.0) + c.get("SQ_INSTS_VALU_TRANS_F64", 0.0))
+            extra[f"{p}_valu_issue_frac"] = issue / (VALU_ISSUE_PER_S * r["kernel_s"])
+            extra[f"{p}_valu_lane_instr_per_pp"] = 64.0 * c["SQ_INSTS_VALU"] / pp
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
         extra["split_vs_f64_errors"] = split_error_report(outs["split"], outs["f64"])
@@ -240,11 +270,22 @@ def main():
         srv.close()
         extra["serving"] = {k: sv[k] for k in ("rate_req_per_s", "duration_s", "requests", "latency_p50_ms",
                                               "latency_p99_ms", "latency_max_ms", "pp_per_s", "precision_counts")}
-        extra["serving"]["latency"] = "entry to sarbp_server_submit -> result in the caller's buffer (queueing included)"
+        extra["serving"]["latency"] = ("scheduled Poisson arrival -> result in the caller's buffer (driver lag, "
+                                       "submission and queueing included)")
         extra["latency_p50_ms"] = sv["latency_p50_ms"]
         extra["latency_p99_ms"] = sv["latency_p99_ms"]
         log(f"serving {args.serve_rate}/s for {args.serve_seconds}s: p50 {sv['latency_p50_ms']:.2f} ms, "
             f"p99 {sv['latency_p99_ms']:.2f} ms, {sv['pp_per_s']:.3e} pp/s")
+        if args.serve_load_rate > 0:  # sustained load: many requests share each round
+            srv = Server()
+            pool = make_pool([sc], nreq_per_dataset=256, seed=2)
+            sl = poisson_run(srv, [ds], pool, args.serve_load_rate, args.serve_seconds, warmup_s=1.0)
+            srv.close()
+            extra["serving_load"] = {k: sl[k] for k in ("rate_req_per_s", "duration_s", "requests", "latency_p50_ms",
+                                                       "latency_p99_ms", "pp_per_s", "rounds", "rounds_per_request",
+                                                       "gpu_busy_frac")}
+            log(f"serving {args.serve_load_rate}/s: p50 {sl['latency_p50_ms']:.2f} ms, p99 {sl['latency_p99_ms']:.2f} ms, "
+                f"{sl['pp_per_s']:.3e} pp/s, {sl['rounds_per_request']:.3f} rounds/request")
 
     cpu = None
     if not args.no_cpu:
@@ -266,16 +307,17 @@ def main():
     kernel_s = r["kernel_s"]
     achieved = BYTES_PER_PP * pp / kernel_s / 1e9
     traffic, traffic_src = None, None
-    tf = os.path.join(ROOT, "profiles", "traffic_full-4000x250x40001.json")
-    kname = {"split": "k_bp_split_tile<0>", "f64": "k_bp_f64_tile"}[head]
-    default_map = (head == "split" and args.split_map == "16,0") or (head == "f64" and args.f64_map == "1,8")
-    if not args.small and default_map and not args.sweep and os.path.exists(tf):
-        with open(tf) as f:
-            tj = json.load(f)
-        if tj.get("pp_per_launch") == pp and kname in tj.get("kernels", {}):
-            traffic = int(tj["kernels"][kname]["traffic_bytes"])
-            traffic_src = ("profiles/traffic_full-4000x250x40001.json: rocprofv3 --pmc FETCH_SIZE x%.3f + WRITE_SIZE x%.3f "
-                           "per launch of %s (scales from a 1 GiB copy probe)" % (tj["fetch_scale"], tj["write_scale"], kname))
+    kname = KERNEL_OF[head]
+    if prof and default_maps(args) and kname in prof.get("kernels", {}):
+        kp = prof["kernels"][kname]
+        traffic = int(kp["traffic_bytes"])
+        traffic_src = ("%s: rocprofv3 --pmc FETCH_SIZE x%.3f + WRITE_SIZE x%.3f per launch of %s (scales from a "
+                       "1 GiB copy probe)" % (PROFILE_JSON, prof["fetch_scale"], prof["write_scale"], kname))
+        # the profiled session's own figures for the same kernel: the trace's average launch and
+        # the roofline fraction it implies, beside this run's HIP-event figure
+        extra["profile"] = {"source": PROFILE_JSON, "kernel": kname, "trace_avg_ms": kp["avg_ns"] / 1e6,
+                            "trace_roofline_frac": BYTES_PER_PP * pp / (kp["avg_ns"] * 1e-9) / 1e9 / HBM_PEAK_GBS,
+                            "session_event_ms": kp.get("bench_event_ms")}
     line = {
         "metric": "pixel·pulses/s split-FP32 vs FP64 at ≤1e-3 rad phase err; p50/p99 request latency",
         "value": r["pp_per_s"],

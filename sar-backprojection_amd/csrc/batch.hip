@@ -133,7 +133,7 @@ void reqbuf_layout(int64_t n64, int state_slots, size_t off[10]) {
 
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b,
                   hipStream_t st, DevicePool *pool, const std::vector<int> *tile_len, float r0min,
-                  const void *packed, char *host_out) {
+                  const void *packed, char *host_out, int narrow) {
   b = ReqBuf();
   if (r.n < 0) return sarbp_fail(SARBP_EINVAL, "negative pixel count");
   if (r.n > 0 && (!r.x || !r.y || !r.z || !r.mask || !r.index || !r.sint))
@@ -180,6 +180,7 @@ int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, in
   b.out = host_out ? (double2 *)host_out : (double2 *)(p + off[6]);
   b.ncnt = host_out ? (int32_t *)(host_out + (off[7] - off[6])) : (int32_t *)(p + off[7]);
   b.state = state_slots > 0 ? (BpState *)(p + off[8]) : nullptr;
+  b.f64_narrow = narrow >= 0 ? narrow != 0 : request_narrow(ds, r);
   b.r0min = r0min >= 0.f ? r0min
                          : r.precision == SARBP_PREC_SPLIT ? sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n) : 0.f;
   const void *src[6] = {r.x, r.y, r.z, r.mask, r.index, r.sint};
@@ -222,6 +223,13 @@ void reqbuf_free(ReqBuf &b) {
   b.base = nullptr;
 }
 
+bool request_narrow(const sarbp_dataset *ds, const sarbp_request &r) {
+  if (r.n <= 0) return true;
+  double rmin = 0.0, rmax = 0.0;
+  return sarbp_live_range_bounds(ds, r.x, r.y, r.z, r.mask, r.sint, r.n, rmin, rmax) &&
+         sarbp_narrow_ranges(ds, rmin, rmax);
+}
+
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
   BatchReqDev d;
   d.g = ds->geom();
@@ -232,6 +240,7 @@ BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b) {
   d.tg = b.tg;
   d.precision = b.precision;
   d.accum = b.accum;
+  d.f64_narrow = b.f64_narrow ? 1 : 0;
   return d;
 }
 

@@ -36,6 +36,7 @@ struct ReqBuf {
   BpState *state = nullptr;
   int64_t pp = 0;                 // pixel-pulse pairs of the request (0 when tile_len was given)
   float r0min = 0.f;              // lower bound of the pixel-to-sensor distances (sarbp_pixel_r0min)
+  bool f64_narrow = false;        // the live pixels' distances satisfy sarbp_narrow_ranges
   std::vector<int> tile_len;      // union length of each tile (pulses)
 };
 
@@ -65,9 +66,13 @@ void reqbuf_layout(int64_t n, int state_slots, size_t off[10]);
 // the block's layout (uploaded with one copy; null: the six arrays of r are copied one by one).
 // host_out: page-locked host memory, already zero, that the kernels write the means (n double2)
 // and, 256-byte aligned after them, the counts (n int32) into, instead of the device block.
+// narrow: whether the FP64 terms may take the narrow-range form (request_narrow; negative:
+// decided here).
 int reqbuf_create(sarbp_dataset *ds, const sarbp_request &r, int state_slots, int accum, ReqBuf &b, hipStream_t st,
                   DevicePool *pool = nullptr, const std::vector<int> *tile_len = nullptr, float r0min = -1.f,
-                  const void *packed = nullptr, char *host_out = nullptr);
+                  const void *packed = nullptr, char *host_out = nullptr, int narrow = -1);
+// sarbp_live_range_bounds and sarbp_narrow_ranges for a request's host pixel arrays.
+bool request_narrow(const sarbp_dataset *ds, const sarbp_request &r);
 void reqbuf_free(ReqBuf &b);
 BatchReqDev reqbuf_dev(const sarbp_dataset *ds, const ReqBuf &b);
 

@@ -54,7 +54,10 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
     split = s.use;  // not a compact tile: FP64 terms (same decision every range)
   }
   if (!split) {
-    f64_span(g, tab, t, p0, p1, ar, ai, cnt);
+    if (r.f64_narrow)
+      f64_span<false>(g, tab, t, p0, p1, ar, ai, cnt);
+    else
+      f64_span<true>(g, tab, t, p0, p1, ar, ai, cnt);
   } else if (r.accum == 0) {
     float sr = (float)ar, si = (float)ai;  // the Kahan running sum is held exactly in the state
     split_span<0>(g, t, s, p0, p1, rec0, C, sr, si, cr, ci, ar, ai, cnt);

@@ -242,6 +242,7 @@ struct BatchReqDev {
   int32_t precision;  // SARBP_PREC_*
   int32_t accum;      // split path accumulation (0 Kahan, 1 FP64 fold)
   int32_t nseg = 1;   // pulse segments per tile (state slots per pixel when > 1)
+  int32_t f64_narrow = 0;  // FP64 terms may take the narrow-range form (sarbp_narrow_ranges)
 };
 
 // Work item: tiles [tile_lo, tile_lo + W) of one request (one per wavefront of the block), pulses

@@ -93,6 +93,7 @@ struct ServerReq {
   int32_t precision = SARBP_PREC_F64;
   double est = 0.0;
   float r0min = 0.f;  // the pixels' distance bound (split requests; from the submission pass)
+  bool narrow = false;  // FP64 terms may take the narrow-range form (from the submission pass)
   ReqBuf buf;
   bool on_device = false;
   int off = 0;       // pulses of every tile's union already focused (multiple of the chunk)
@@ -280,7 +281,7 @@ bool sarbp_server::admit(ServerReq *r) {
   // the uploads are ordered before the round's launch on the same stream; only a saved state,
   // whose host copy is dropped below, needs the stream drained first
   int rc = reqbuf_create(r->ds, r->req, r->nseg, default_tiling(r->precision).accum, r->buf, st_up, &pool, &r->tile_len,
-                         r->r0min, r->host, r->host ? r->host + r->out_off : nullptr);
+                         r->r0min, r->host, r->host ? r->host + r->out_off : nullptr, r->narrow ? 1 : 0);
   if (rc) return false;
   if (r->off == 0) drop_saved(r);  // no progress to restore
   if (!r->saved.empty()) {
@@ -663,6 +664,7 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
     SplitRows rows;
     const bool split = req->precision == SARBP_PREC_SPLIT;
     request_tiles(ds, r->req, tg, r->tile_len, r->pp, split ? &rows : nullptr);
+    r->narrow = request_narrow(ds, r->req);
     if (split) {
       r->est = split_phase_estimate(ds, rows);
       r->r0min = (float)rows.r0min;
