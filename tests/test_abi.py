@@ -118,8 +118,9 @@ def test_product_path_fails_loudly_without_a_device():
 
 
 def test_integration_stub_matches_the_header():
-    """The ctypes stub INTEGRATION.md gives a reference maintainer declares the entry points it binds
-    with the header's parameter counts and loads against the built library (declarations only)."""
+    """The ctypes stub that the integration guide at the repository root gives a reference
+    maintainer declares the entry points it binds with the header's parameter counts and loads
+    against the built library (declarations only)."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     block = text.split("## 3.")[1].split("```python")[1].split("```")[0]
     decl = block.split("def focus_gpu")[0].replace('ctypes.CDLL("libsarbp.so")', "ctypes.CDLL(LIB)")
