@@ -250,10 +250,11 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 // branch in the hot loop).
 __device__ __forceinline__ float2 split_term(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
                                              float beta_t) {
-  // turns of the pixel's phase beyond the reference's: the product rounds once (|c1 dR| < ~100 turns:
-  // at most 4e-6 turns, 2.4e-5 rad, and uncorrelated from pulse to pulse) and is reduced exactly
-  const float ph = c1 * dR;
-  const float tt = __builtin_amdgcn_fractf(ph) + fmaf(beta_t, dR * dR, t0);
+  // the phase in turns: the reference's (t0, in [0, 1)) plus the linear term in one fused
+  // multiply-add, which rounds once (|c1 dR| < ~100 turns: at most 4e-6 turns, 2.4e-5 rad, and
+  // uncorrelated from pulse to pulse) and is reduced exactly, then the quadratic term
+  const float ph = fmaf(c1, dR, t0);
+  const float tt = fmaf(beta_t * dR, dR, __builtin_amdgcn_fractf(ph));
   const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
   const float cs = __builtin_amdgcn_cosf(tt);
   const float w = fmaf(two_r0, dR, r0sq);  // R^2 = R0^2 + 2 R0 dR (dR^2 / R0^2 < 1e-7: below FP32's resolution of R^2)
