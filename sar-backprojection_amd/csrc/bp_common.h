@@ -35,6 +35,25 @@ __device__ __forceinline__ double sqrt_rn(double x) {
 // memory (same values, same arithmetic).  Larger |x| (slant ranges beyond ~80 km at lambda =
 // 3 cm) take the library sincos (WIDE = false: the caller guarantees |x| < 2^25, and the library
 // path and its registers are left out).
+// The two halves of the table method for |x| < 2^25: sincos_reduce finds r and the table entry
+// (S, C) = (sin, cos)(j h), sincos_finish combines them; callers may issue the table read early and
+// finish later.
+__device__ __forceinline__ void sincos_reduce(double x, const double2 *tab, double2 &e, double &r) {
+  constexpr double INV_H = 40.743665431525205957;            // 128/pi
+  constexpr double H_HI = 3.141592653589793116 / 128.0;      // pi rounded to double, / 128
+  constexpr double H_LO = 1.2246467991473531772e-16 / 128.0; // (pi - that) / 128
+  const double n = __builtin_rint(x * INV_H);
+  e = tab[(int)n & (SARBP_SCTAB - 1)];  // (sin, cos)(j h)
+  r = fma(-n, H_LO, fma(-n, H_HI, x));
+}
+__device__ __forceinline__ void sincos_finish(double2 e, double r, double &sn, double &cs) {
+  const double r2 = r * r;
+  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  sn = fma(e.x, cr, e.y * sr);
+  cs = fma(e.y, cr, -(e.x * sr));
+}
+
 template <bool WIDE = true>
 __device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double &sn, double &cs) {
   if constexpr (WIDE)
@@ -42,17 +61,10 @@ __device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double 
       sincos(x, &sn, &cs);
       return;
     }
-  constexpr double INV_H = 40.743665431525205957;            // 128/pi
-  constexpr double H_HI = 3.141592653589793116 / 128.0;      // pi rounded to double, / 128
-  constexpr double H_LO = 1.2246467991473531772e-16 / 128.0; // (pi - that) / 128
-  const double n = __builtin_rint(x * INV_H);
-  const double2 e = tab[(int)n & (SARBP_SCTAB - 1)];  // (sin, cos)(j h)
-  const double r = fma(-n, H_LO, fma(-n, H_HI, x));
-  const double r2 = r * r;
-  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
-  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
-  sn = fma(e.x, cr, e.y * sr);
-  cs = fma(e.y, cr, -(e.x * sr));
+  double2 e;
+  double r;
+  sincos_reduce(x, tab, e, r);
+  sincos_finish(e, r, sn, cs);
 }
 
 // rint((r - near) / dra), the range bin of sar_focuser_puntual.py:100-101, from a product with

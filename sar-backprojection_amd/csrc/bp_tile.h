@@ -89,20 +89,29 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, co
     // every lane issues its gather (a lane without a term reads RAW_NO_SAMPLE: zero, no memory
     // access), so the load sits on no branch and the wait before its use counts exactly the one
     // gather issued after it
-    auto issue = [&](int a, const float2 *colp, double4 tr, bool &ok, double &r, double &r2, float2 &d) {
+    // the phase's argument reduction and its sin/cos table read are issued here too, so that the
+    // LDS read is also in flight over the other pulse's work
+    auto issue = [&](int a, const float2 *colp, double4 tr, bool &ok, double &rr, double &r2, float2 &d,
+                     double2 &e) {
       const __amdgpu_buffer_rsrc_t col = raw_column_at(colp, g.nra);  // uniform
-      double q = 0.0;
+      double q = 0.0, r = 0.0;
       // the range and bin of every lane, on no branch (lanes outside their aperture are masked
       // off in the instructions either way)
       const bool in_bin = f64_bin<false>(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
       ok = a < p1 && a >= t.lo && a < t.hi && in_bin;
       const double qs = ok ? q : 0.0;  // in [bin_lo, nra-1) when ok
       d = raw_load(col, ok ? (unsigned)(int)qs << 3 : RAW_NO_SAMPLE);
+      double ph;
+      {
+#pragma clang fp contract(off)
+        ph = g.kphase * r + g.beta * r2;  // f64_phase's argument
+      }
+      sincos_reduce(ph, tab, e, rr);  // |ph| < 2^24 (narrow ranges); any table index is in bounds
     };
-    auto consume = [&](bool ok, double r, double r2, float2 d) {
+    auto consume = [&](bool ok, double rr, double r2, float2 d, double2 e) {
       if (ok) {
         double sn, cs;
-        f64_phase<false>(g, tab, r, r2, sn, cs);
+        sincos_finish(e, rr, sn, cs);
         f64_acc(r2, sn, cs, d, ar, ai);
         ++cnt;
       }
@@ -113,16 +122,17 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, co
     bool okA, okB;
     double rA = 0.0, r2A = 0.0, rB = 0.0, r2B = 0.0;
     float2 dA, dB;
-    issue(p0, colA, tp[0], okA, rA, r2A, dA);
+    double2 eA, eB;
+    issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
     double4 t1 = tp[1], t2 = tp[2];
     for (int a = p0; a < p1; a += 2) {  // wave-uniform pulses a (set A) and a+1 (set B)
-      issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB);
+      issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB, eB);
       const double4 t3 = tp[3];
-      consume(okA, rA, r2A, dA);
+      consume(okA, rA, r2A, dA, eA);
       colA += 2 * cstride;
-      issue(a + 2, colA, t2, okA, rA, r2A, dA);
+      issue(a + 2, colA, t2, okA, rA, r2A, dA, eA);
       const double4 t4 = tp[4];
-      consume(okB, rB, r2B, dB);
+      consume(okB, rB, r2B, dB, eB);
       tp += 2;
       t1 = t3;
       t2 = t4;
