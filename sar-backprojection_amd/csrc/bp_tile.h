@@ -390,10 +390,32 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
     Z.d = raw_load(c2r, oZ | s.poison);
     W.d = raw_load(c3r, oW | s.poison);
   } else {
-    split_first<FOURTH>(g, t, s, rk, a, c0r, inv_dra, nbin_bytes, X);
-    split_first<FOURTH>(g, t, s, rk + rstep, a + 1, c1r, inv_dra, nbin_bytes, Y);
-    split_first<FOURTH>(g, t, s, rk + 2 * rstep, a + 2, c2r, inv_dra, nbin_bytes, Z);
-    split_first<FOURTH>(g, t, s, rk + 3 * rstep, a + 3, c3r, inv_dra, nbin_bytes, W);
+    // per-lane aperture and swath tests; the near-tie bins of the four pulses again on one branch
+    // (entered when any lane is near a tie; inside, only lanes within their aperture re-decide)
+    unsigned oX, oY, oZ, oW;
+    float fX, fY, fZ, fW;
+    split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
+    split_bin<FOURTH>(s, rk + rstep, inv_dra, Y, oY, fY);
+    split_bin<FOURTH>(s, rk + 2 * rstep, inv_dra, Z, oZ, fZ);
+    split_bin<FOURTH>(s, rk + 3 * rstep, inv_dra, W, oW, fW);
+    const bool aX = a >= t.lo && a < t.hi, aY = a + 1 >= t.lo && a + 1 < t.hi, aZ = a + 2 >= t.lo && a + 2 < t.hi,
+               aW = a + 3 >= t.lo && a + 3 < t.hi;
+    if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
+      __asm__ volatile("");  // keeps the branch: near-tie bins are rare
+      if (aX && fX > s.guard) oX = exact_off(g, t, a);
+      if (aY && fY > s.guard) oY = exact_off(g, t, a + 1);
+      if (aZ && fZ > s.guard) oZ = exact_off(g, t, a + 2);
+      if (aW && fW > s.guard) oW = exact_off(g, t, a + 3);
+    }
+    const bool kX = oX < nbin_bytes, kY = oY < nbin_bytes, kZ = oZ < nbin_bytes, kW = oW < nbin_bytes;
+    X.bad = aX && !kX;  // bin_lo <= bin < Nra-1 fails inside the aperture: not counted
+    Y.bad = aY && !kY;
+    Z.bad = aZ && !kZ;
+    W.bad = aW && !kW;
+    X.d = raw_load(c0r, aX && kX ? oX : SPLIT_NO_SAMPLE);
+    Y.d = raw_load(c1r, aY && kY ? oY : SPLIT_NO_SAMPLE);
+    Z.d = raw_load(c2r, aZ && kZ ? oZ : SPLIT_NO_SAMPLE);
+    W.d = raw_load(c3r, aW && kW ? oW : SPLIT_NO_SAMPLE);
   }
   const float2 tx = split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
   const float2 ty = split_term(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
