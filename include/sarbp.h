@@ -252,6 +252,7 @@ typedef struct sarbp_server_stats {
    * building rounds (work lists, per-XCD queues, their upload and the launches), and rounds from
    * their start to the end of their wait for the device */
   double admit_s, build_s, round_wall_s;
+  int64_t records;  /* request records held (unreleased requests) */
 } sarbp_server_stats;
 
 int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
@@ -262,7 +263,7 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out);
  * for a split request whose tile rows x pulse_chunk records exceed the batch kernel's LDS.  SARBP_EBUDGET if the request alone exceeds the
  * budget.  The request's latency (sarbp_request_info: done_s - submit_s) starts at the entry to
  * this call.  Once a request is done or failed the server keeps only its record (state, timings,
- * counts): its copy of the inputs is released. */
+ * counts; until sarbp_server_release): its copy of the inputs is released. */
 int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request *req, double *out_c128,
                         int32_t *ncontrib, int64_t *id);
 int sarbp_server_poll(sarbp_server *s, int64_t id, int32_t *state);
@@ -276,6 +277,11 @@ int sarbp_server_preempt(sarbp_server *s, int64_t id);
 /* Queue a preempted request again; it continues from its saved state. */
 int sarbp_server_resume(sarbp_server *s, int64_t id);
 int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info);
+/* Drop the record of a done or failed request (its state, timings and counts): afterwards its id
+ * is unknown.  SARBP_ESTATE while the request is queued, running, preempted, or waited on by
+ * sarbp_server_wait in another thread.  A server that is never asked to release keeps one record
+ * per request until it is destroyed. */
+int sarbp_server_release(sarbp_server *s, int64_t id);
 int sarbp_server_get_stats(sarbp_server *s, sarbp_server_stats *st);
 /* Finish the current round, fail the requests still queued or running, join the thread. */
 int sarbp_server_destroy(sarbp_server *s);

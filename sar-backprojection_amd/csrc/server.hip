@@ -110,6 +110,7 @@ struct ServerReq {
   bool holds_ds = false;  // counted in ds->server_refs until done or failed
   double submit_s = 0, admit_s = -1, done_s = -1, round_s = -1, gpu_s = -1;
   int32_t rounds = 0, preemptions = 0;
+  int32_t waiters = 0;  // threads in sarbp_server_wait on this request (the record must outlive them)
   int64_t pp = 0;
   int nseg = 1;     // pulse segments per tile (> 1: focused whole in its first round)
   int seg_len = 0;  // pulses per segment (a multiple of the chunk)
@@ -778,11 +779,15 @@ int sarbp_server_wait(sarbp_server *s, int64_t id, double timeout_s) {
   if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
   ServerReq *r = it->second.get();
   auto fin = [r] { return r->state == SARBP_REQ_DONE || r->state == SARBP_REQ_FAILED || r->state == SARBP_REQ_PREEMPTED; };
+  r->waiters++;  // sarbp_server_release leaves the record alone meanwhile
+  bool ok = true;
   if (timeout_s < 0) {
     s->cv_done.wait(lk, fin);
-  } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), fin)) {
-    return SARBP_EAGAIN;
+  } else {
+    ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), fin);
   }
+  r->waiters--;
+  if (!ok) return SARBP_EAGAIN;
   if (r->state == SARBP_REQ_FAILED) return sarbp_fail(SARBP_EDEVICE, "request failed");
   if (r->state == SARBP_REQ_PREEMPTED) return sarbp_fail(SARBP_ESTATE, "request is preempted");
   return SARBP_OK;
@@ -842,6 +847,19 @@ int sarbp_server_info(sarbp_server *s, int64_t id, sarbp_request_info *info) {
   return SARBP_OK;
 }
 
+int sarbp_server_release(sarbp_server *s, int64_t id) {
+  if (!s) return sarbp_fail(SARBP_EINVAL, "null handle");
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return sarbp_fail(SARBP_EINVAL, "unknown request id");
+  const ServerReq *r = it->second.get();
+  if ((r->state != SARBP_REQ_DONE && r->state != SARBP_REQ_FAILED) || r->completing)
+    return sarbp_fail(SARBP_ESTATE, "request is not finished");
+  if (r->waiters > 0) return sarbp_fail(SARBP_ESTATE, "request is being waited on");
+  s->reqs.erase(it);  // done or failed: no queue, round or completion list refers to it any more
+  return SARBP_OK;
+}
+
 int sarbp_server_get_stats(sarbp_server *s, sarbp_server_stats *st) {
   if (!s || !st) return sarbp_fail(SARBP_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
@@ -849,6 +867,7 @@ int sarbp_server_get_stats(sarbp_server *s, sarbp_server_stats *st) {
   st->device_bytes_in_use = s->bytes_in_use;
   st->active = (int32_t)s->active.size();
   st->queued = (int32_t)s->queue.size();
+  st->records = (int64_t)s->reqs.size();
   return SARBP_OK;
 }
 

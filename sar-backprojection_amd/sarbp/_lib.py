@@ -61,7 +61,7 @@ class ServerStats(ctypes.Structure):
     """sarbp_server_stats (include/sarbp.h)."""
     _fields_ = [("submitted", _i64), ("completed", _i64), ("failed", _i64), ("rounds", _i64), ("pp_done", _i64),
                 ("gpu_busy_s", _d), ("device_bytes_in_use", _i64), ("active", _i32), ("queued", _i32),
-                ("admit_s", _d), ("build_s", _d), ("round_wall_s", _d)]
+                ("admit_s", _d), ("build_s", _d), ("round_wall_s", _d), ("records", _i64)]
 
 
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_PREEMPTED, REQ_FAILED = 0, 1, 2, 3, 4
@@ -102,6 +102,7 @@ SIGNATURES = {
     "sarbp_server_resume": (ctypes.c_int, [_p, _i64]),
     "sarbp_server_info": (ctypes.c_int, [_p, _i64, ctypes.POINTER(RequestInfo)]),
     "sarbp_server_get_stats": (ctypes.c_int, [_p, ctypes.POINTER(ServerStats)]),
+    "sarbp_server_release": (ctypes.c_int, [_p, _i64]),
     "sarbp_server_destroy": (ctypes.c_int, [_p]),
 }
 

@@ -28,7 +28,8 @@ from .batch import Request
 class Handle:
     """A submitted request.  It holds the dataset, the request's arrays and the result buffers the
     library writes into; the server keeps every handle of an unfinished request alive, so dropping
-    one before completion is safe."""
+    one before completion is safe.  When the last reference to the handle of a finished request
+    goes, the server's record of it is released (sarbp_server_release)."""
 
     def __init__(self, server, rid, ds, req: Request, out, cnt, t_submit):
         self.server, self.id, self.ds, self.req = server, rid, ds, req
@@ -66,6 +67,14 @@ class Handle:
 
     def result(self, with_counts=False):
         return (self.out, self.cnt) if with_counts else self.out
+
+    def __del__(self):
+        try:  # at interpreter exit the module's globals may already be gone
+            h = getattr(self.server, "_h", None)
+            if h and _lib.alive():
+                _lib.load().sarbp_server_release(h, self.id)  # refused (ignored) unless finished
+        except Exception:
+            pass
 
 
 class Server:

@@ -287,3 +287,29 @@ def test_split_request_beyond_lds_is_refused_alone(gpu, world):
         assert h.wait(timeout=60)
     assert srv.stats()["failed"] == 0
     srv.close()
+
+
+def test_release_request_records(gpu, world):
+    """A finished request's record is dropped by sarbp_server_release (its id becomes unknown); an
+    unfinished one's release is refused; a Handle dropped after completion releases its record."""
+    import gc
+    sc, ds, _ = world
+    reqs = patch_requests(sc, 3, seed=13, w_range=(64, 128), h_range=(32, 64), sint_range=(2000, 3000))
+    srv = Server(quantum_ms=0.001, pulse_chunk=8, max_segments=1)  # many rounds: still running below
+    L = _lib.load()
+    hs = [srv.submit(ds, r) for r in reqs]
+    st = L.sarbp_server_release(srv._h, hs[0].id)
+    assert st == _lib.ESTATE, st  # queued or running
+    assert all(h.wait(timeout=120) for h in hs)
+    assert srv.stats()["records"] == 3
+    assert L.sarbp_server_release(srv._h, hs[1].id) == _lib.OK
+    with pytest.raises(ValueError, match="unknown request id"):
+        hs[1].info()
+    assert L.sarbp_server_release(srv._h, hs[1].id) == _lib.EINVAL  # unknown id now
+    assert srv.stats()["records"] == 2
+    want = hs[2].result().copy()
+    del hs[2]
+    gc.collect()
+    assert srv.stats()["records"] == 1  # released with its handle
+    assert np.all(np.isfinite(want))
+    srv.close()
