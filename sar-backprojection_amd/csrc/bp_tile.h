@@ -357,11 +357,10 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
 // Groups start at multiples of 4 from the chunk start, and chunks and caller ranges start at
 // multiples of the chunk length (a multiple of 8) from the union start, so the grouping -- and the
 // result -- does not depend on how a caller splits the pulse range.
-template <int ACCUM, bool FOURTH, bool FAST>
-__device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                            const PulseRec *rk, int rstep, int a, const float2 *col,
-                                            float inv_dra, unsigned nbin_bytes, float beta_t, float &sr, float &si,
-                                            float &cr, float &ci, int32_t &cnt) {
+template <bool FOURTH, bool FAST>
+__device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
+                                              const PulseRec *rk, int rstep, int a, const float2 *col,
+                                              float inv_dra, unsigned nbin_bytes, float beta_t, int32_t &cnt) {
   const int64_t nra = g.nra;  // col: bin bin_lo of the column of pulse a
   const __amdgpu_buffer_rsrc_t c0r = raw_split_column(col, nbin_bytes), c1r = raw_split_column(col + nra, nbin_bytes),
                                c2r = raw_split_column(col + 2 * nra, nbin_bytes),
@@ -421,13 +420,12 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
   const float2 ty = split_term(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
   const float2 tz = split_term(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
   const float2 tw = split_term(W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
-  // pulses a .. a+3 (a a multiple of 4 from the chunk start) are summed pairwise, then the group
-  // sum is accumulated with compensation
-  split_acc<ACCUM>(make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y)), sr, si, cr, ci);
   if (!FAST && (X.bad || Y.bad || Z.bad || W.bad)) {  // rare: a bin outside the swath
     __asm__ volatile("");  // keeps the branch (the count is not worth computing on every group)
     cnt -= (X.bad ? 1 : 0) + (Y.bad ? 1 : 0) + (Z.bad ? 1 : 0) + (W.bad ? 1 : 0);
   }
+  // pulses a .. a+3 (a a multiple of 4 from the chunk start) summed pairwise
+  return make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y));
 }
 
 // Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
@@ -471,15 +469,25 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row;
     const float2 *col = g.raw + (int64_t)c0 * g.nra + g.bin_lo;  // bin bin_lo of the column of pulse c0 + k
-    int k = 0;
-    for (; k + 3 < cn; k += 4, rk += 4 * rstep, col += 4 * g.nra) {
-      const int a = c0 + k;
+    // one group of four pulses in the fast or the general form
+    auto group = [&](int a, const PulseRec *rg, const float2 *cg) -> float2 {
       if (swath && a >= s.core_lo && a + 3 < s.core_hi)
-        split_group<ACCUM, FOURTH, true>(g, t, s, rk, rstep, a, col, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci,
-                                         cnt);
-      else
-        split_group<ACCUM, FOURTH, false>(g, t, s, rk, rstep, a, col, inv_dra, nbin_bytes, beta_t, sr, si, cr, ci,
-                                          cnt);
+        return split_group<FOURTH, true>(g, t, s, rg, rstep, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
+      return split_group<FOURTH, false>(g, t, s, rg, rstep, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
+    };
+    int k = 0;
+    // eight pulses per compensated step: the sums of two groups (a multiple of 8 from the chunk
+    // start) are added, then accumulated
+    for (; k + 7 < cn; k += 8, rk += 8 * rstep, col += 8 * g.nra) {
+      const float2 s1 = group(c0 + k, rk, col);
+      const float2 s2 = group(c0 + k + 4, rk + 4 * rstep, col + 4 * g.nra);
+      split_acc<ACCUM>(make_float2(s1.x + s2.x, s1.y + s2.y), sr, si, cr, ci);
+    }
+    if (k + 3 < cn) {  // a last group of four (only at the end of the pulse range)
+      split_acc<ACCUM>(group(c0 + k, rk, col), sr, si, cr, ci);
+      k += 4;
+      rk += 4 * rstep;
+      col += 4 * g.nra;
     }
     for (; k < cn; ++k, rk += rstep, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
