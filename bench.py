@@ -154,8 +154,9 @@ def target_peaks(sc, img_split, img_f64):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=30,
+                    help="untimed passes first (the GPU's clocks settle over the first ~0.1 s of load)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--strip", type=int, default=4000, help="pixel columns of the all-cores CPU baseline strip "
                     "(4000: the whole scene, ~3.5 s on 16 cores)")
