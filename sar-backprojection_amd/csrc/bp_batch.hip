@@ -95,7 +95,8 @@ __device__ __forceinline__ void batch_tile(const BatchReqDev &r, const BatchItem
   }
 }
 
-// C pulses of records per refill, up to max_rows rows per tile: C * max_rows records per wavefront.
+// C pulses of records per refill, up to max_rows rows per tile: (C + 1) * max_rows records per
+// wavefront (rows C + 1 records apart).
 template <int MODE>
 __device__ __forceinline__ void batch_loop(const BatchReqDev *__restrict__ reqs, const BatchItem *__restrict__ items,
                                            const int32_t *__restrict__ seg, int32_t *heads, int C, int max_rows) {
@@ -103,7 +104,7 @@ __device__ __forceinline__ void batch_loop(const BatchReqDev *__restrict__ reqs,
   __shared__ double2 tab[SARBP_SCTAB];  // every request's table holds the same values
   load_sctab(reqs[0].g, tab);
   const int wave = threadIdx.x >> 6;
-  const int rec0 = wave * C * max_rows;  // this wavefront's records in sarbp_lds_rec
+  const int rec0 = wave * (C + 1) * max_rows;  // this wavefront's records in sarbp_lds_rec
   const int home = blockIdx.x % SARBP_BATCH_QUEUES;
   for (int qi = 0; qi < SARBP_BATCH_QUEUES;) {
     const int q = (home + qi) % SARBP_BATCH_QUEUES;
@@ -237,7 +238,7 @@ int sarbp_launch_combine(const BpState *part, int nseg, int64_t n, double2 *out,
 }
 
 size_t sarbp_batch_lds_bytes(int waves, int chunk, int max_rows) {
-  return (size_t)waves * (size_t)chunk * (size_t)max_rows * sizeof(PulseRec);
+  return (size_t)waves * (size_t)(chunk + 1) * (size_t)max_rows * sizeof(PulseRec);
 }
 
 int sarbp_batch_resident_blocks(int device, int mode, int waves, int chunk, int max_rows, int *blocks) {

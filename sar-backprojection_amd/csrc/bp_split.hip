@@ -5,7 +5,8 @@
 
 namespace {
 
-// LDS records staged per refill: C pulses x TY rows = SPLIT_SCENE_RECS (6 KiB per wavefront).
+// LDS records staged per refill: C pulses x TY rows = SPLIT_SCENE_RECS, rows C + 1 records apart
+// (SPLIT_SCENE_RECS + TY records, about 6.2 KiB per wavefront).
 constexpr int SPLIT_SCENE_RECS = 128;
 
 // Five wavefronts per SIMD (at most 96 VGPRs): the loops fit without spills (measured on the
@@ -41,10 +42,10 @@ int sarbp_launch_split(const sarbp_dataset *ds, const sarbp_scene *sc, const Til
   const int64_t blocks = (tg.ntiles() + SARBP_NUM_XCD - 1) / SARBP_NUM_XCD * SARBP_NUM_XCD;
   if (blocks > 0x7fffffff) return sarbp_fail(SARBP_EINVAL, "too many tiles for one launch");
   if (t.accum == 0)
-    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<0>, dim3((unsigned)blocks), dim3(64), (SPLIT_SCENE_RECS + tg.ty) * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   else
-    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), SPLIT_SCENE_RECS * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
+    hipLaunchKernelGGL(k_bp_split_tile<1>, dim3((unsigned)blocks), dim3(64), (SPLIT_SCENE_RECS + tg.ty) * sizeof(PulseRec), st, sc->geom(), sc->view(), tg,
                        sc->out, sc->ncnt);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "split kernel launch");

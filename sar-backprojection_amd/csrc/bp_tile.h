@@ -155,9 +155,12 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, co
   }
 }
 
+// 1.5 * 2^23: adding it rounds an FP32 value of magnitude below 2^22 to an integer.
+constexpr float SPLIT_ROUND_MAGIC = 12582912.0f;
+
 struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads)
   float4 a;        // ux, uy, uz, 1/(2R0)
-  float4 b;        // 1/(2R0^2), f0 = q0 - B0, 8*B0 (int bits: byte offset of bin B0), t0 = frac(phi0/2pi)
+  float4 b;        // 1/(2R0^2), f0 = q0 - B0, int bits: byte offset of bin B0 less 8 bits(magic), t0 = frac(phi0/2pi)
   float4 c;        // c1 = (k + 2 beta R0)/2pi, R0^2, 2R0, 1/(8R0^3)
 };
 
@@ -185,7 +188,9 @@ __device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, d
   const double c1 = (g.kphase + 2.0 * g.beta * r0) * INV_2PI;
   r.a = make_float4((float)(dx * inv), (float)(dy * inv), (float)(dz * inv), (float)(0.5 * inv));
   // byte offset of bin B0 from bin bin_lo, where the split path's column resources start
-  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0), __int_as_float(((int)b0 - g.bin_lo) * 8), (float)t0);
+  r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0),
+                    __uint_as_float((unsigned)((int)b0 - g.bin_lo) * 8u - 8u * (unsigned)__float_as_int(SPLIT_ROUND_MAGIC)),
+                    (float)t0);
   r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
   return (int)b0;
 }
@@ -243,9 +248,9 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   // the series truncation
   const double trunc = split_truncation_m((double)s.omax, (double)r0min);
   s.guard = 0.5f - (2e-5f + 4e-7f * s.omax * (float)g.inv_dra) - (float)(fmin(trunc, 1.0) * g.inv_dra);
-  // compact tiles only, whose series truncation stays far below the FP32 rounding; and |dR|/dra well
-  // inside the int32 byte offsets of split_first
-  s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 16777216.0 &&
+  // compact tiles only, whose series truncation stays far below the FP32 rounding; and |dR|/dra below
+  // 2^21 bins, so that |f0 + dR/dra| < 2^22 for the rounding of split_bin
+  s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 2097152.0 &&
           trunc * __builtin_fabs(g.kphase) <= SPLIT_TRUNC_MAX_RAD;
   s.poison = act ? 0u : 0x80000000u;
   s.core_lo = __builtin_amdgcn_readfirstlane(wave_max_i32(act ? t.lo : INT_MIN));
@@ -322,9 +327,13 @@ __device__ __forceinline__ void split_bin(const SplitTile &s, const PulseRec *rk
   const float dR = fmaf(t2, c, av);
   h.dR = dR;
   const float qb = fmaf(dR, inv_dra, rb.y);
-  const float rq = __builtin_rintf(qb);
-  // byte offset of bin B0 + rq (|bin| < 2^26: no wrap-around)
-  off = ((unsigned)(int)rq << 3) + (unsigned)__float_as_int(rb.z);
+  // rint(qb) for |qb| < 2^22: qb + 1.5 * 2^23 lies in [2^23, 2^24), where the sum's rounding (to
+  // nearest, ties to even) keeps the integer part and the low mantissa bits hold rint(qb) +
+  // 0x400000; both differences below are exact
+  const float qm = qb + SPLIT_ROUND_MAGIC;
+  const float rq = qm - SPLIT_ROUND_MAGIC;
+  // byte offset of bin B0 + rq: rb.z holds 8 (B0 - bin_lo) - 8 bits(SPLIT_ROUND_MAGIC), modulo 2^32
+  off = ((unsigned)__float_as_int(qm) << 3) + (unsigned)__float_as_int(rb.z);
   fr = fabsf(qb - rq);
 }
 
@@ -359,7 +368,7 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
 // result -- does not depend on how a caller splits the pulse range.
 template <bool FOURTH, bool FAST>
 __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                              const PulseRec *rk, int rstep, int a, const float2 *col,
+                                              const PulseRec *rk, int a, const float2 *col,
                                               float inv_dra, unsigned nbin_bytes, float beta_t, int32_t &cnt) {
   const int64_t nra = g.nra;  // col: bin bin_lo of the column of pulse a
   const __amdgpu_buffer_rsrc_t c0r = raw_split_column(col, nbin_bytes), c1r = raw_split_column(col + nra, nbin_bytes),
@@ -373,9 +382,9 @@ __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t
     unsigned oX, oY, oZ, oW;
     float fX, fY, fZ, fW;
     split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
-    split_bin<FOURTH>(s, rk + rstep, inv_dra, Y, oY, fY);
-    split_bin<FOURTH>(s, rk + 2 * rstep, inv_dra, Z, oZ, fZ);
-    split_bin<FOURTH>(s, rk + 3 * rstep, inv_dra, W, oW, fW);
+    split_bin<FOURTH>(s, rk + 1, inv_dra, Y, oY, fY);
+    split_bin<FOURTH>(s, rk + 2, inv_dra, Z, oZ, fZ);
+    split_bin<FOURTH>(s, rk + 3, inv_dra, W, oW, fW);
     if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
       __asm__ volatile("");  // keeps the branch: near-tie bins are rare
       if (fX > s.guard) oX = exact_off(g, t, a);
@@ -394,9 +403,9 @@ __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t
     unsigned oX, oY, oZ, oW;
     float fX, fY, fZ, fW;
     split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
-    split_bin<FOURTH>(s, rk + rstep, inv_dra, Y, oY, fY);
-    split_bin<FOURTH>(s, rk + 2 * rstep, inv_dra, Z, oZ, fZ);
-    split_bin<FOURTH>(s, rk + 3 * rstep, inv_dra, W, oW, fW);
+    split_bin<FOURTH>(s, rk + 1, inv_dra, Y, oY, fY);
+    split_bin<FOURTH>(s, rk + 2, inv_dra, Z, oZ, fZ);
+    split_bin<FOURTH>(s, rk + 3, inv_dra, W, oW, fW);
     const bool aX = a >= t.lo && a < t.hi, aY = a + 1 >= t.lo && a + 1 < t.hi, aZ = a + 2 >= t.lo && a + 2 < t.hi,
                aW = a + 3 >= t.lo && a + 3 < t.hi;
     if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
@@ -429,9 +438,11 @@ __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t
 }
 
 // Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
-// and staged records do not depend on where a caller splits the range.  The wavefront's C * TY
-// LDS records start at rec0: record rec0 + k*TY + r for pulse c0+k and row r (lane l stages
-// records l + 64*m, all of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for ACCUM 1
+// and staged records do not depend on where a caller splits the range.  The wavefront's
+// TY * (C + 1) LDS records start at rec0: record rec0 + r*(C+1) + k for pulse c0+k and row r, so a
+// lane reads its row's consecutive pulses at constant offsets, and the four rows' records of one
+// pulse (C + 1 records = 12 (C + 1) dwords apart) fall in different LDS banks (lane l stages the
+// records of pulses l / TY + 64/TY * m of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for ACCUM 1
 // (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.  Groups of four
 // pulses inside every aperture of the tile, in a chunk whose reference bins all lie well inside
 // the swath, take the FAST form; both forms give the same terms.
@@ -445,7 +456,8 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
   const float beta_t = (float)(g.beta * INV_2PI);
   const unsigned nbin_bytes = (unsigned)(g.nra - 1 - g.bin_lo) * 8u;
   const int nrec = C << s.ty_shift;
-  const int rstep = 1 << s.ty_shift;
+  const int rows_mask = (1 << s.ty_shift) - 1;
+  const int rs = C + 1;  // row stride in records
   const int blo = g.bin_lo + s.margin, bhi = (int)(g.nra - 1) - 1 - s.margin;  // reference bins inside the swath
   // contributors: the pulses of this lane's aperture within [c0, p1), less the rare ones whose bin
   // falls outside the swath (counted as they occur)
@@ -458,7 +470,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     for (int m = lane; m < nrec; m += 64) {
       const int p = c0 + (m >> s.ty_shift);
       if (p < p1) {
-        const int b0 = make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + m]);
+        const int b0 = make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + (m & rows_mask) * rs + (m >> s.ty_shift)]);
         inside = inside && b0 >= blo && b0 <= bhi;
       }
     }
@@ -467,29 +479,29 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const int cn = min(C, p1 - c0);
-    const PulseRec *rk = rec + rec0 + s.row;
+    const PulseRec *rk = rec + rec0 + s.row * rs;
     const float2 *col = g.raw + (int64_t)c0 * g.nra + g.bin_lo;  // bin bin_lo of the column of pulse c0 + k
     // one group of four pulses in the fast or the general form
     auto group = [&](int a, const PulseRec *rg, const float2 *cg) -> float2 {
       if (swath && a >= s.core_lo && a + 3 < s.core_hi)
-        return split_group<FOURTH, true>(g, t, s, rg, rstep, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
-      return split_group<FOURTH, false>(g, t, s, rg, rstep, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
+        return split_group<FOURTH, true>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
+      return split_group<FOURTH, false>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
     };
     int k = 0;
     // eight pulses per compensated step: the sums of two groups (a multiple of 8 from the chunk
     // start) are added, then accumulated
-    for (; k + 7 < cn; k += 8, rk += 8 * rstep, col += 8 * g.nra) {
+    for (; k + 7 < cn; k += 8, rk += 8, col += 8 * g.nra) {
       const float2 s1 = group(c0 + k, rk, col);
-      const float2 s2 = group(c0 + k + 4, rk + 4 * rstep, col + 4 * g.nra);
+      const float2 s2 = group(c0 + k + 4, rk + 4, col + 4 * g.nra);
       split_acc<ACCUM>(make_float2(s1.x + s2.x, s1.y + s2.y), sr, si, cr, ci);
     }
     if (k + 3 < cn) {  // a last group of four (only at the end of the pulse range)
       split_acc<ACCUM>(group(c0 + k, rk, col), sr, si, cr, ci);
       k += 4;
-      rk += 4 * rstep;
+      rk += 4;
       col += 4 * g.nra;
     }
-    for (; k < cn; ++k, rk += rstep, col += g.nra) {  // tail (only at the end of the pulse range)
+    for (; k < cn; ++k, ++rk, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
       split_first<FOURTH>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
       split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);

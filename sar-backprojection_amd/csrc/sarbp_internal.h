@@ -275,7 +275,7 @@ inline int sarbp_batch_mode(bool any_f64, bool any_split) {
 }
 
 // Launch the persistent batch kernel: `waves` wavefronts per block (1, 2 or 4), pulse chunk C,
-// tiles of at most max_rows rows (LDS holds C * max_rows pulse records per wavefront).
+// tiles of at most max_rows rows (LDS holds (C + 1) * max_rows pulse records per wavefront).
 // d_heads: SARBP_HEADS_BYTES, zeroed by the launcher; d_seg: SARBP_BATCH_QUEUES + 1 item offsets;
 // nitems: the number of items; mode: SARBP_BATCH_* covering the requests' precisions.  grid_blocks <= 0 sizes the grid from the occupancy, at most one
 // block per item.
