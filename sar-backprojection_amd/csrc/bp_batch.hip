@@ -179,7 +179,10 @@ __global__ __launch_bounds__(256) void k_bp_combine(const BpState *__restrict__ 
 namespace {
 
 // One request per blockIdx.y; the blocks along x stride over its slots (clear) or pixels (combine).
+// The combine's time is its result writes into the requests' host blocks, so it keeps few blocks per
+// request and leaves the CUs to the batch kernel that runs beside it.
 constexpr int SEG_BLOCKS_X = 64;
+constexpr int COMBINE_BLOCKS_X = 8;
 
 __global__ __launch_bounds__(256) void k_segments_clear(const BatchReqDev *__restrict__ reqs,
                                                         const int32_t *__restrict__ list) {
@@ -221,7 +224,7 @@ int sarbp_launch_segments_clear(const BatchReqDev *d_reqs, const int32_t *d_list
 int sarbp_launch_segments_combine(const BatchReqDev *d_reqs, const int32_t *d_list, int count, hipStream_t st) {
   if (count <= 0) return SARBP_OK;
   if (count > 65535) return sarbp_fail(SARBP_EINVAL, "too many segmented requests for one launch");
-  hipLaunchKernelGGL(k_segments_combine, dim3(SEG_BLOCKS_X, (unsigned)count), dim3(256), 0, st, d_reqs, d_list);
+  hipLaunchKernelGGL(k_segments_combine, dim3(COMBINE_BLOCKS_X, (unsigned)count), dim3(256), 0, st, d_reqs, d_list);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return sarbp_hip_fail(e, "segment combine launch");
   return SARBP_OK;

@@ -147,7 +147,9 @@ struct Round {
   bool failed = false;  // a launch or copy of it failed
   bool timed = false;   // e0 and e1 bracket its kernels
   ItemBuffers ib;
-  hipEvent_t e0 = nullptr, e1 = nullptr, done = nullptr;  // done: after its result copies
+  hipEvent_t e0 = nullptr, e1 = nullptr;  // around its batch kernel (and the slot clear before it)
+  hipEvent_t ek = nullptr;                 // after its batch kernel
+  hipEvent_t done = nullptr;               // after its segments' combine: every result written
 };
 
 struct sarbp_server {
@@ -170,12 +172,14 @@ struct sarbp_server {
   bool started = false;
   int device = 0;
   int64_t slots_f64 = 4096, slots_split = 4096;  // resident wavefronts of the FP64 / split batch kernels
-  // two streams at the default priority: rounds' kernels (st) and admissions' uploads (st_up), so
-  // that the uploads of the next round's requests overlap this round's kernels.  Nothing but copies
-  // goes on st_up: a kernel there would wait for the persistent round kernel to leave the CUs, and
-  // the copies queued behind it with it.  The results need no copy: the kernels write them into the
-  // requests' page-locked blocks.
-  hipStream_t st = nullptr, st_up = nullptr;
+  // three streams at the default priority: rounds' kernels (st), admissions' uploads (st_up), so
+  // that the uploads of the next round's requests overlap this round's kernels, and the segmented
+  // requests' combine (st_comb).  Nothing but copies goes on st_up: a kernel there would wait for
+  // the persistent round kernel to leave the CUs, and the copies queued behind it with it.  The
+  // results need no copy: the kernels write them into the requests' page-locked blocks; the
+  // combine, whose time is those writes over the host link, runs beside the next round's batch
+  // kernel (it needs few registers and no LDS, so its blocks fit beside the persistent blocks).
+  hipStream_t st = nullptr, st_up = nullptr, st_comb = nullptr;
   hipEvent_t ev_up = nullptr;  // after the uploads of the requests admitted last
   bool up_pending = false;     // the next round waits for ev_up
   Round rounds[2];
@@ -193,11 +197,11 @@ struct sarbp_server {
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
   void sync_all() {
-    for (hipStream_t q : {st, st_up})
+    for (hipStream_t q : {st, st_up, st_comb})
       if (q) (void)hipStreamSynchronize(q);
   }
   void destroy_streams() {
-    for (hipStream_t *q : {&st, &st_up})
+    for (hipStream_t *q : {&st, &st_up, &st_comb})
       if (*q) {
         (void)hipStreamDestroy(*q);
         *q = nullptr;
@@ -207,7 +211,7 @@ struct sarbp_server {
     if (ev_up) (void)hipEventDestroy(ev_up);
     ev_up = nullptr;
     for (Round &R : rounds)
-      for (hipEvent_t *ev : {&R.e0, &R.e1, &R.done})
+      for (hipEvent_t *ev : {&R.e0, &R.e1, &R.ek, &R.done})
         if (*ev) {
           (void)hipEventDestroy(*ev);
           *ev = nullptr;
@@ -404,12 +408,14 @@ void sarbp_server::launch_round(Round &R) {
     if (!rc && e == hipSuccess)
       rc = sarbp_launch_batch(device, R.ib.d_reqs, R.ib.d_items, R.ib.d_seg, R.ib.d_heads, (int)items.size(),
                               sarbp_batch_mode(R.any_f64, R.any_split), R.waves, chunk, R.max_rows, 0, st);
-    if (!rc && e == hipSuccess)
-      rc = sarbp_launch_segments_combine(R.ib.d_reqs, R.ib.d_list, (int)segmented.size(), st);
     R.timed = !rc && e == hipSuccess;
   }
-  if (!rc && e == hipSuccess) e = hipEventRecord(R.e1, st);  // after the round's kernels
-  if (!rc && e == hipSuccess) e = hipEventRecord(R.done, st);
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.e1, st);  // after the round's batch kernel
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.ek, st);
+  if (!rc && e == hipSuccess) e = hipStreamWaitEvent(st_comb, R.ek, 0);
+  if (!rc && e == hipSuccess && !items.empty())
+    rc = sarbp_launch_segments_combine(R.ib.d_reqs, R.ib.d_list, (int)segmented.size(), st_comb);
+  if (!rc && e == hipSuccess) e = hipEventRecord(R.done, st_comb);
   R.failed = rc || e != hipSuccess;
   R.live = true;
 }
@@ -621,10 +627,12 @@ int sarbp_server_create(const sarbp_server_config *cfg, sarbp_server **out) {
   }
   if (e == hipSuccess) e = hipStreamCreate(&s->st);
   if (e == hipSuccess) e = hipStreamCreate(&s->st_up);
+  if (e == hipSuccess) e = hipStreamCreate(&s->st_comb);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_up, hipEventDisableTiming);
   for (Round &R : s->rounds) {
     if (e == hipSuccess) e = hipEventCreate(&R.e0);
     if (e == hipSuccess) e = hipEventCreate(&R.e1);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&R.ek, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&R.done, hipEventDisableTiming);
   }
   if (e != hipSuccess) {
