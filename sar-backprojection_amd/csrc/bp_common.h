@@ -42,14 +42,29 @@ __device__ __forceinline__ void sincos_reduce(double x, const double2 *tab, doub
   constexpr double INV_H = 40.743665431525205957;            // 128/pi
   constexpr double H_HI = 3.141592653589793116 / 128.0;      // pi rounded to double, / 128
   constexpr double H_LO = 1.2246467991473531772e-16 / 128.0; // (pi - that) / 128
-  const double n = __builtin_rint(x * INV_H);
-  e = tab[(int)n & (SARBP_SCTAB - 1)];  // (sin, cos)(j h)
+  // n = the integer nearest x / h, from one fused multiply-add: x * (128/pi) + 1.5 * 2^52 (|x| < 2^25,
+  // so the sum lies in [2^52, 2^53), where its rounding keeps the integer part); the low word of
+  // the sum holds n modulo 2^32, the table index
+  constexpr double MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+  const double nm = fma(x, INV_H, MAGIC);
+  const double n = nm - MAGIC;
+  e = tab[(int)(unsigned)__builtin_bit_cast(unsigned long long, nm) & (SARBP_SCTAB - 1)];  // (sin, cos)(j h)
   r = fma(-n, H_LO, fma(-n, H_HI, x));
 }
+// fma(a, b, c) with all three operands in vector registers: the same IEEE fused multiply-add as
+// fma(), written so that the compiler cannot turn it into the two-address form whose accumulator
+// would be a per-use copy of the constant c (b and c are loop-invariant constants here, kept in
+// registers across the loop)
+__device__ __forceinline__ double fma_vvv(double a, double b, double c) {
+  double d;
+  __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
 __device__ __forceinline__ void sincos_finish(double2 e, double r, double &sn, double &cs) {
   const double r2 = r * r;
-  const double sr = fma(r * r2, fma(r2, 1.0 / 120.0, -1.0 / 6.0), r);
-  const double cr = fma(r2, fma(r2, fma(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
+  const double sr = fma(r * r2, fma_vvv(r2, 1.0 / 120.0, -1.0 / 6.0), r);
+  const double cr = fma(r2, fma(r2, fma_vvv(r2, -1.0 / 720.0, 1.0 / 24.0), -0.5), 1.0);
   sn = fma(e.x, cr, e.y * sr);
   cs = fma(e.y, cr, -(e.x * sr));
 }
@@ -77,6 +92,19 @@ __device__ __forceinline__ double range_bin(const BpGeom &g, double r) {
   const double q = __builtin_rint(qe);
   if (__builtin_fabs(qe - q) > fma(__builtin_fabs(qe), -0x1p-49, 0.5 - 0x1p-40)) return __builtin_rint(num / g.dra);
   return q;
+}
+
+// Range bin (as a double), R^2 and R of one pixel-pulse pair of process_point
+// (sar_focuser_puntual.py:82-106), without the swath test.
+__device__ __forceinline__ void f64_range(const BpGeom &g, double x, double y, double z, double tx, double ty,
+                                          double tz, double &q, double &r2, double &r) {
+#pragma clang fp contract(off)
+  const double dx = x - tx;                  // :82-84
+  const double dy = y - ty;
+  const double dz = z - tz;
+  r2 = dx * dx + dy * dy + dz * dz;          // :88
+  r = sqrt_rn<false>(r2);                    // :94
+  q = range_bin(g, r);                       // :100-101, half to even
 }
 
 // Range bin (as a double), R^2 and R of one pixel-pulse pair of process_point

@@ -38,6 +38,7 @@ struct TileLane {
   bool valid = false;  // lane maps to a pixel of the grid
   double x = 0, y = 0, z = 0;
   int lo = 0x7fffffff, hi = -0x7fffffff;  // own pulses [lo, hi); empty when lo >= hi
+  int len = 0;                            // hi - lo, or 0 when empty
   int wlo = 0, whi = 0;                   // union over the wavefront (uniform)
 };
 
@@ -56,6 +57,7 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
       if (h > l) {
         t.lo = (int)l;
         t.hi = (int)h;
+        t.len = (int)(h - l);
       }
       t.x = px.x[t.i];
       t.y = px.y[t.i];
@@ -97,10 +99,14 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, co
       double q = 0.0, r = 0.0;
       // the range and bin of every lane, on no branch (lanes outside their aperture are masked
       // off in the instructions either way)
-      const bool in_bin = f64_bin<false>(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
-      ok = a < p1 && a >= t.lo && a < t.hi && in_bin;
-      const double qs = ok ? q : 0.0;  // in [bin_lo, nra-1) when ok
-      d = raw_load(col, ok ? (unsigned)(int)qs << 3 : RAW_NO_SAMPLE);
+      f64_range(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
+      // the aperture and swath tests of f64_bin as unsigned integer compares: q is finite here
+      // (narrow ranges), and a q beyond the int32 range converts to INT_MIN or INT_MAX, which
+      // lies outside [bin_lo, nra-1) either way (nra <= 2^24)
+      const int iq = (int)q;
+      ok = a < p1 && (unsigned)(a - t.lo) < (unsigned)t.len &&
+           (unsigned)(iq - g.bin_lo) < (unsigned)(g.nra - 1 - g.bin_lo);
+      d = raw_load(col, ok ? (unsigned)iq << 3 : RAW_NO_SAMPLE);
       double ph;
       {
 #pragma clang fp contract(off)
