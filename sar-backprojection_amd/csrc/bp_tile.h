@@ -209,7 +209,6 @@ struct SplitTile {
   float ox, oy, oz, qo;  // offset of this lane's pixel from its row's reference point
   float omax;            // largest pixel-to-reference distance in the tile (uniform)
   bool use;              // the tile is compact enough for the split form (uniform)
-  unsigned poison;       // 0 for a lane with a non-empty aperture, else SPLIT_NO_SAMPLE
   int core_lo, core_hi;  // pulses inside every non-empty aperture of the tile (uniform; may be empty)
   int margin;            // bins: a reference bin this far inside [0, Nra-1) keeps every pixel's bin inside
   float guard;           // |f0 + dR/dra - nearest integer| above this -> decide the bin in FP64
@@ -258,7 +257,6 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   // 2^21 bins, so that |f0 + dR/dra| < 2^22 for the rounding of split_bin
   s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 2097152.0 &&
           trunc * __builtin_fabs(g.kphase) <= SPLIT_TRUNC_MAX_RAD;
-  s.poison = act ? 0u : 0x80000000u;
   s.core_lo = __builtin_amdgcn_readfirstlane(wave_max_i32(act ? t.lo : INT_MIN));
   s.core_hi = __builtin_amdgcn_readfirstlane(wave_min_i32(act ? t.hi : INT_MAX));
   // |bin - B0| <= |f0| + |dR|/dra + 1 (an exact re-decision) with |dR| <= 1.01 |o|
@@ -266,11 +264,11 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   return s;
 }
 
-// Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi); exactly
-// zero when the pulse does not contribute, whose sample d is zero (computed on every lane: no
-// branch in the hot loop).
-__device__ __forceinline__ float2 split_term(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
-                                             float beta_t) {
+// Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi), added to
+// acc by four fused multiply-adds (adds nothing when the pulse does not contribute: its sample d is
+// zero).  Computed on every lane: no branch in the hot loop.
+__device__ __forceinline__ void split_mac(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
+                                          float beta_t, float2 &acc) {
   // the phase in turns: the reference's (t0, in [0, 1)) plus the linear term in one fused
   // multiply-add, which rounds once (|c1 dR| < ~100 turns: at most 4e-6 turns, 2.4e-5 rad, and
   // uncorrelated from pulse to pulse) and is reduced exactly, then the quadratic term
@@ -280,7 +278,8 @@ __device__ __forceinline__ float2 split_term(float dR, float2 d, float t0, float
   const float cs = __builtin_amdgcn_cosf(tt);
   const float w = fmaf(two_r0, dR, r0sq);  // R^2 = R0^2 + 2 R0 dR (dR^2 / R0^2 < 1e-7: below FP32's resolution of R^2)
   const float wr = w * d.x, wi = w * d.y;
-  return make_float2(fmaf(wr, cs, wi * sn), fmaf(wi, cs, -(wr * sn)));
+  acc.x = fmaf(wi, sn, fmaf(wr, cs, acc.x));   // + Re(w d) cos + Im(w d) sin
+  acc.y = fmaf(-wr, sn, fmaf(wi, cs, acc.y));  // + Im(w d) cos - Re(w d) sin
 }
 
 // Compensated accumulation of one (pair) sum: Kahan (ACCUM 0) or plain FP32 within a chunk (ACCUM 1).
@@ -368,59 +367,56 @@ __device__ __forceinline__ void split_first(const BpGeom &g, const TileLane &t, 
 }
 
 // Pulses are processed four at a time: the four gathers are in flight before any term is formed;
-// the terms are summed pairwise and the group sum is accumulated, groups in ascending pulse order.
-// Groups start at multiples of 4 from the chunk start, and chunks and caller ranges start at
-// multiples of the chunk length (a multiple of 8) from the union start, so the grouping -- and the
-// result -- does not depend on how a caller splits the pulse range.
+// the terms are added to acc in ascending pulse order.  Groups start at multiples of 4 from the
+// chunk start, and chunks and caller ranges start at multiples of the chunk length (a multiple of
+// 8) from the union start, so the grouping -- and the result -- does not depend on how a caller
+// splits the pulse range.
+//
+// FAST: every lane with an aperture is inside it for the four pulses and every bin is inside the
+// swath (the caller checks both), so no lane tests either; a lane without an aperture reads a sample
+// at its row's reference point (its offsets are zero), in range or zero, and is never counted or
+// stored (its count is zero, so its mean is written as 0).  Otherwise each lane tests its aperture
+// and its bin against the swath, and a bin outside the swath inside the aperture is not counted.
+// Either way the rare near-tie bins of the four pulses are decided on one branch.
 template <bool FOURTH, bool FAST>
-__device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
-                                              const PulseRec *rk, int a, const float2 *col,
-                                              float inv_dra, unsigned nbin_bytes, float beta_t, int32_t &cnt) {
+__device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
+                                            const PulseRec *rk, int a, const float2 *col, float inv_dra,
+                                            unsigned nbin_bytes, float beta_t, float2 &acc, int32_t &cnt) {
   const int64_t nra = g.nra;  // col: bin bin_lo of the column of pulse a
   const __amdgpu_buffer_rsrc_t c0r = raw_split_column(col, nbin_bytes), c1r = raw_split_column(col + nra, nbin_bytes),
                                c2r = raw_split_column(col + 2 * nra, nbin_bytes),
                                c3r = raw_split_column(col + 3 * nra, nbin_bytes);
   SplitHalf X, Y, Z, W;
-  if (FAST) {
-    // every lane with an aperture is inside it for the four pulses and every bin is inside the
-    // swath (the caller checks both): no per-lane tests, and the rare near-tie bins of the four
-    // pulses are decided on one branch
-    unsigned oX, oY, oZ, oW;
-    float fX, fY, fZ, fW;
-    split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
-    split_bin<FOURTH>(s, rk + 1, inv_dra, Y, oY, fY);
-    split_bin<FOURTH>(s, rk + 2, inv_dra, Z, oZ, fZ);
-    split_bin<FOURTH>(s, rk + 3, inv_dra, W, oW, fW);
-    if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
-      __asm__ volatile("");  // keeps the branch: near-tie bins are rare
+  unsigned oX, oY, oZ, oW;
+  float fX, fY, fZ, fW;
+  split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
+  split_bin<FOURTH>(s, rk + 1, inv_dra, Y, oY, fY);
+  split_bin<FOURTH>(s, rk + 2, inv_dra, Z, oZ, fZ);
+  split_bin<FOURTH>(s, rk + 3, inv_dra, W, oW, fW);
+  // inside the lane's aperture (unsigned compare: an empty aperture has len 0)
+  const bool aX = (unsigned)a - (unsigned)t.lo < (unsigned)t.len, aY = (unsigned)(a + 1) - (unsigned)t.lo < (unsigned)t.len,
+             aZ = (unsigned)(a + 2) - (unsigned)t.lo < (unsigned)t.len, aW = (unsigned)(a + 3) - (unsigned)t.lo < (unsigned)t.len;
+  if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
+    __asm__ volatile("");  // keeps the branch: near-tie bins are rare
+    if (!FAST) {  // only lanes within their aperture re-decide
+      if (aX && fX > s.guard) oX = exact_off(g, t, a);
+      if (aY && fY > s.guard) oY = exact_off(g, t, a + 1);
+      if (aZ && fZ > s.guard) oZ = exact_off(g, t, a + 2);
+      if (aW && fW > s.guard) oW = exact_off(g, t, a + 3);
+    } else {
       if (fX > s.guard) oX = exact_off(g, t, a);
       if (fY > s.guard) oY = exact_off(g, t, a + 1);
       if (fZ > s.guard) oZ = exact_off(g, t, a + 2);
       if (fW > s.guard) oW = exact_off(g, t, a + 3);
     }
-    X.bad = Y.bad = Z.bad = W.bad = false;
-    X.d = raw_load(c0r, oX | s.poison);  // beyond the column for a lane without an aperture
-    Y.d = raw_load(c1r, oY | s.poison);
-    Z.d = raw_load(c2r, oZ | s.poison);
-    W.d = raw_load(c3r, oW | s.poison);
+  }
+  X.bad = Y.bad = Z.bad = W.bad = false;
+  if (FAST) {
+    X.d = raw_load(c0r, oX);
+    Y.d = raw_load(c1r, oY);
+    Z.d = raw_load(c2r, oZ);
+    W.d = raw_load(c3r, oW);
   } else {
-    // per-lane aperture and swath tests; the near-tie bins of the four pulses again on one branch
-    // (entered when any lane is near a tie; inside, only lanes within their aperture re-decide)
-    unsigned oX, oY, oZ, oW;
-    float fX, fY, fZ, fW;
-    split_bin<FOURTH>(s, rk, inv_dra, X, oX, fX);
-    split_bin<FOURTH>(s, rk + 1, inv_dra, Y, oY, fY);
-    split_bin<FOURTH>(s, rk + 2, inv_dra, Z, oZ, fZ);
-    split_bin<FOURTH>(s, rk + 3, inv_dra, W, oW, fW);
-    const bool aX = a >= t.lo && a < t.hi, aY = a + 1 >= t.lo && a + 1 < t.hi, aZ = a + 2 >= t.lo && a + 2 < t.hi,
-               aW = a + 3 >= t.lo && a + 3 < t.hi;
-    if (fmaxf(fmaxf(fX, fY), fmaxf(fZ, fW)) > s.guard) {
-      __asm__ volatile("");  // keeps the branch: near-tie bins are rare
-      if (aX && fX > s.guard) oX = exact_off(g, t, a);
-      if (aY && fY > s.guard) oY = exact_off(g, t, a + 1);
-      if (aZ && fZ > s.guard) oZ = exact_off(g, t, a + 2);
-      if (aW && fW > s.guard) oW = exact_off(g, t, a + 3);
-    }
     const bool kX = oX < nbin_bytes, kY = oY < nbin_bytes, kZ = oZ < nbin_bytes, kW = oW < nbin_bytes;
     X.bad = aX && !kX;  // bin_lo <= bin < Nra-1 fails inside the aperture: not counted
     Y.bad = aY && !kY;
@@ -431,16 +427,14 @@ __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t
     Z.d = raw_load(c2r, aZ && kZ ? oZ : SPLIT_NO_SAMPLE);
     W.d = raw_load(c3r, aW && kW ? oW : SPLIT_NO_SAMPLE);
   }
-  const float2 tx = split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t);
-  const float2 ty = split_term(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t);
-  const float2 tz = split_term(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t);
-  const float2 tw = split_term(W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t);
+  split_mac(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, acc);
+  split_mac(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t, acc);
+  split_mac(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t, acc);
+  split_mac(W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t, acc);
   if (!FAST && (X.bad || Y.bad || Z.bad || W.bad)) {  // rare: a bin outside the swath
     __asm__ volatile("");  // keeps the branch (the count is not worth computing on every group)
     cnt -= (X.bad ? 1 : 0) + (Y.bad ? 1 : 0) + (Z.bad ? 1 : 0) + (W.bad ? 1 : 0);
   }
-  // pulses a .. a+3 (a a multiple of 4 from the chunk start) summed pairwise
-  return make_float2((tx.x + ty.x) + (tz.x + tw.x), (tx.y + ty.y) + (tz.y + tw.y));
 }
 
 // Split terms for pulses [c0, p1), c0 on the chunk grid of the union (wlo + C*k), so chunk sums
@@ -451,7 +445,7 @@ __device__ __forceinline__ float2 split_group(const BpGeom &g, const TileLane &t
 // records of pulses l / TY + 64/TY * m of row l % TY).  Sums: Kahan (sr, si, cr, ci) for ACCUM 0; for ACCUM 1
 // (sr, si) hold the running chunk sum, folded into (ar, ai) at each chunk end.  Groups of four
 // pulses inside every aperture of the tile, in a chunk whose reference bins all lie well inside
-// the swath, take the FAST form; both forms give the same terms.
+// the swath, take the fast form; both forms give the same terms.
 template <int ACCUM, bool FOURTH>
 __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t, const SplitTile &s, int c0,
                                              int p1, int rec0, int C, float &sr, float &si, float &cr,
@@ -471,13 +465,20 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     const int lo = max(c0, t.lo), hi = min(p1, t.hi);  // compared, not subtracted: an empty aperture is
     cnt += hi > lo ? hi - lo : 0;                       // (INT_MAX, -INT_MAX)
   }
+  // lane l stages the records m = l + 64 j: pulse c0 + (m >> ty_shift) of row m & rows_mask, so
+  // from one j to the next the pulse advances by 64 >> ty_shift and the row stays the same
+  const int pstep = 64 >> s.ty_shift;
+  PulseRec *const srec = rec + rec0 + (lane & rows_mask) * rs + (lane >> s.ty_shift);
   for (; c0 < p1; c0 += C) {
     bool inside = true;
-    for (int m = lane; m < nrec; m += 64) {
-      const int p = c0 + (m >> s.ty_shift);
-      if (p < p1) {
-        const int b0 = make_rec(g, s.sxc, s.syc, s.szc, g.traj[p], rec[rec0 + (m & rows_mask) * rs + (m >> s.ty_shift)]);
-        inside = inside && b0 >= blo && b0 <= bhi;
+    {
+      PulseRec *sr = srec;
+      const double4 *tp = g.traj + c0 + (lane >> s.ty_shift);
+      for (int m = lane, p = c0 + (lane >> s.ty_shift); m < nrec; m += 64, p += pstep, sr += pstep, tp += pstep) {
+        if (p < p1) {
+          const int b0 = make_rec(g, s.sxc, s.syc, s.szc, *tp, *sr);
+          inside = inside && b0 >= blo && b0 <= bhi;
+        }
       }
     }
     const bool swath = __builtin_amdgcn_ballot_w64(!inside) == 0;  // uniform
@@ -487,22 +488,26 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     const int cn = min(C, p1 - c0);
     const PulseRec *rk = rec + rec0 + s.row * rs;
     const float2 *col = g.raw + (int64_t)c0 * g.nra + g.bin_lo;  // bin bin_lo of the column of pulse c0 + k
-    // one group of four pulses in the fast or the general form
-    auto group = [&](int a, const PulseRec *rg, const float2 *cg) -> float2 {
+    // one group of four pulses in the fast or the general form, added to acc
+    auto group = [&](int a, const PulseRec *rg, const float2 *cg, float2 &acc) {
       if (swath && a >= s.core_lo && a + 3 < s.core_hi)
-        return split_group<FOURTH, true>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
-      return split_group<FOURTH, false>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, cnt);
+        split_group<FOURTH, true>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, acc, cnt);
+      else
+        split_group<FOURTH, false>(g, t, s, rg, a, cg, inv_dra, nbin_bytes, beta_t, acc, cnt);
     };
     int k = 0;
-    // eight pulses per compensated step: the sums of two groups (a multiple of 8 from the chunk
-    // start) are added, then accumulated
+    // eight pulses per compensated step: the terms of two groups (a multiple of 8 from the chunk
+    // start) are summed by fused multiply-adds in pulse order, then the sum is accumulated
     for (; k + 7 < cn; k += 8, rk += 8, col += 8 * g.nra) {
-      const float2 s1 = group(c0 + k, rk, col);
-      const float2 s2 = group(c0 + k + 4, rk + 4, col + 4 * g.nra);
-      split_acc<ACCUM>(make_float2(s1.x + s2.x, s1.y + s2.y), sr, si, cr, ci);
+      float2 acc = make_float2(0.f, 0.f);
+      group(c0 + k, rk, col, acc);
+      group(c0 + k + 4, rk + 4, col + 4 * g.nra, acc);
+      split_acc<ACCUM>(acc, sr, si, cr, ci);
     }
     if (k + 3 < cn) {  // a last group of four (only at the end of the pulse range)
-      split_acc<ACCUM>(group(c0 + k, rk, col), sr, si, cr, ci);
+      float2 acc = make_float2(0.f, 0.f);
+      group(c0 + k, rk, col, acc);
+      split_acc<ACCUM>(acc, sr, si, cr, ci);
       k += 4;
       rk += 4;
       col += 4 * g.nra;
@@ -510,7 +515,9 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
     for (; k < cn; ++k, ++rk, col += g.nra) {  // tail (only at the end of the pulse range)
       SplitHalf X;
       split_first<FOURTH>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
-      split_acc<ACCUM>(split_term(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t), sr, si, cr, ci);
+      float2 acc = make_float2(0.f, 0.f);
+      split_mac(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, acc);
+      split_acc<ACCUM>(acc, sr, si, cr, ci);
       cnt -= X.bad ? 1 : 0;
     }
     if (ACCUM == 1) {
