@@ -84,13 +84,15 @@ __device__ __forceinline__ void sincos_tab(double x, const double2 *tab, double 
 
 // rint((r - near) / dra), the range bin of sar_focuser_puntual.py:100-101, from a product with
 // 1/dra; the quotient is divided exactly only when the product lies within its error bound of a
-// half-integer, where the two roundings could differ.
+// half-integer, where the two roundings could differ.  The product's error is below 2^-49 |qe| +
+// 2^-40 (the rounding of 1/dra and of the product), under 2^-24 for |qe| < 2^24; beyond that both
+// roundings are bins >= 2^24 - 1, outside every swath (nra <= 2^24), so one constant bound serves.
 __device__ __forceinline__ double range_bin(const BpGeom &g, double r) {
 #pragma clang fp contract(off)
   const double num = r - g.near_range;
   const double qe = num * g.inv_dra;
   const double q = __builtin_rint(qe);
-  if (__builtin_fabs(qe - q) > fma(__builtin_fabs(qe), -0x1p-49, 0.5 - 0x1p-40)) return __builtin_rint(num / g.dra);
+  if (__builtin_fabs(qe - q) > 0.5 - 0x1p-24) return __builtin_rint(num / g.dra);
   return q;
 }
 

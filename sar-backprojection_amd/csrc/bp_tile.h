@@ -40,6 +40,8 @@ struct TileLane {
   int lo = 0x7fffffff, hi = -0x7fffffff;  // own pulses [lo, hi); empty when lo >= hi
   int len = 0;                            // hi - lo, or 0 when empty
   int wlo = 0, whi = 0;                   // union over the wavefront (uniform)
+  int clo = 0, chi = 0;                   // pulses inside every non-empty aperture of the tile (uniform;
+                                          // empty when clo >= chi)
 };
 
 __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView &px, const TileGrid &tg,
@@ -66,83 +68,98 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
   }
   t.wlo = __builtin_amdgcn_readfirstlane(wave_min_i32(t.lo));
   t.whi = __builtin_amdgcn_readfirstlane(wave_max_i32(t.hi));
+  t.clo = __builtin_amdgcn_readfirstlane(wave_max_i32(t.len > 0 ? t.lo : INT_MIN));
+  t.chi = __builtin_amdgcn_readfirstlane(wave_min_i32(t.len > 0 ? t.hi : INT_MAX));
   return t;
 }
 
-// FP64 terms for pulses [p0, p1) of the union (tab: the sin/cos table).  The trajectory sample is loaded one pulse ahead
-// (scalar loads), and the RawData gather is issued as soon as the bin is known, before the phase
-// and its sin/cos, which cover part of its latency.
-//
-// WIDE = false (the narrow-range variant, which has registers to spare): software-pipelined one
-// pulse deeper -- pulse a+1's range, bin and gather are issued before pulse a's phase, sin/cos and
-// accumulation, so the gather's latency is covered by a whole pulse of arithmetic.  The terms and
-// their order into the sums are the same.
+// FP64 terms of the narrow-range variant for pulses [p0, p1) of the union (tab: the sin/cos table),
+// software-pipelined two pulses deep: pulse a+1's range, bin and gather are issued before pulse a's
+// phase, sin/cos and accumulation, so the gather's latency is covered by a whole pulse of
+// arithmetic.  CORE: [p0, p1) lies inside every non-empty aperture of the tile, so a lane tests only
+// that it has an aperture, not where the pulse lies in it.
+template <bool CORE>
+__device__ __forceinline__ void f64_narrow_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0,
+                                                int p1, double &ar, double &ai, int32_t &cnt) {
+  if (p0 >= p1) return;
+  // two register sets, A and B, alternate: while one pulse's gather is in flight the other
+  // pulse's phase and sums are computed.  Column and trajectory pointers advance by addition
+  // (the device trajectory holds SARBP_TRAJ_PAD samples past the last pulse, so reading up to
+  // four pulses ahead needs no clamp; a column past the last pulse is only addressed with
+  // RAW_NO_SAMPLE, which the range check turns into zero without an access).
+  //
+  // every lane issues its gather (a lane without a term reads RAW_NO_SAMPLE: zero, no memory
+  // access), so the load sits on no branch and the wait before its use counts exactly the one
+  // gather issued after it; the phase's argument reduction and its sin/cos table read are issued
+  // there too, so that the LDS read is also in flight over the other pulse's work
+  const bool act = t.len > 0;
+  auto issue = [&](int a, const float2 *colp, double4 tr, bool &ok, double &rr, double &r2, float2 &d,
+                   double2 &e) {
+    const __amdgpu_buffer_rsrc_t col = raw_column_at(colp, g.nra);  // uniform
+    double q = 0.0, r = 0.0;
+    // the range and bin of every lane, on no branch (lanes outside their aperture are masked
+    // off in the instructions either way)
+    f64_range(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
+    // the aperture and swath tests of f64_bin as unsigned integer compares: q is finite here
+    // (narrow ranges), and a q beyond the int32 range converts to INT_MIN or INT_MAX, which
+    // lies outside [bin_lo, nra-1) either way (nra <= 2^24)
+    const int iq = (int)q;
+    const bool in_ap = CORE ? act : (unsigned)(a - t.lo) < (unsigned)t.len;
+    ok = a < p1 && in_ap && (unsigned)(iq - g.bin_lo) < (unsigned)(g.nra - 1 - g.bin_lo);
+    d = raw_load(col, ok ? (unsigned)iq << 3 : RAW_NO_SAMPLE);
+    double ph;
+    {
+#pragma clang fp contract(off)
+      ph = g.kphase * r + g.beta * r2;  // f64_phase's argument
+    }
+    sincos_reduce(ph, tab, e, rr);  // |ph| < 2^24 (narrow ranges); any table index is in bounds
+  };
+  auto consume = [&](bool ok, double rr, double r2, float2 d, double2 e) {
+    if (ok) {
+      double sn, cs;
+      sincos_finish(e, rr, sn, cs);
+      f64_acc(r2, sn, cs, d, ar, ai);
+      ++cnt;
+    }
+  };
+  const int64_t cstride = g.nra;
+  const float2 *colA = g.raw + (int64_t)p0 * cstride;  // pulse a's column (set A)
+  const double4 *tp = g.traj + p0;                     // tp[k]: pulse a + k
+  bool okA, okB;
+  double rA = 0.0, r2A = 0.0, rB = 0.0, r2B = 0.0;
+  float2 dA, dB;
+  double2 eA, eB;
+  issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
+  double4 t1 = tp[1], t2 = tp[2];
+  for (int a = p0; a < p1; a += 2) {  // wave-uniform pulses a (set A) and a+1 (set B)
+    issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB, eB);
+    const double4 t3 = tp[3];
+    consume(okA, rA, r2A, dA, eA);
+    colA += 2 * cstride;
+    issue(a + 2, colA, t2, okA, rA, r2A, dA, eA);
+    const double4 t4 = tp[4];
+    consume(okB, rB, r2B, dB, eB);
+    tp += 2;
+    t1 = t3;
+    t2 = t4;
+  }
+}
+
+// FP64 terms for pulses [p0, p1) of the union (tab: the sin/cos table), in ascending pulse order.
+// The trajectory sample is loaded one pulse ahead (scalar loads), and the RawData gather is issued
+// as soon as the bin is known, before the phase and its sin/cos, which cover part of its latency.
+// WIDE = false (the narrow-range variant, which has registers to spare) takes f64_narrow_span, the
+// pulses inside every aperture of the tile without the per-lane aperture test; the terms and their
+// order into the sums are the same.
 template <bool WIDE = true>
 __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0, int p1,
                                          double &ar, double &ai, int32_t &cnt) {
   if (p0 >= p1) return;
   if constexpr (!WIDE) {
-    // two register sets, A and B, alternate: while one pulse's gather is in flight the other
-    // pulse's phase and sums are computed.  Column and trajectory pointers advance by addition
-    // (the device trajectory holds SARBP_TRAJ_PAD samples past the last pulse, so reading up to
-    // four pulses ahead needs no clamp; a column past the last pulse is only addressed with
-    // RAW_NO_SAMPLE, which the range check turns into zero without an access).
-    //
-    // every lane issues its gather (a lane without a term reads RAW_NO_SAMPLE: zero, no memory
-    // access), so the load sits on no branch and the wait before its use counts exactly the one
-    // gather issued after it
-    // the phase's argument reduction and its sin/cos table read are issued here too, so that the
-    // LDS read is also in flight over the other pulse's work
-    auto issue = [&](int a, const float2 *colp, double4 tr, bool &ok, double &rr, double &r2, float2 &d,
-                     double2 &e) {
-      const __amdgpu_buffer_rsrc_t col = raw_column_at(colp, g.nra);  // uniform
-      double q = 0.0, r = 0.0;
-      // the range and bin of every lane, on no branch (lanes outside their aperture are masked
-      // off in the instructions either way)
-      f64_range(g, t.x, t.y, t.z, tr.x, tr.y, tr.z, q, r2, r);
-      // the aperture and swath tests of f64_bin as unsigned integer compares: q is finite here
-      // (narrow ranges), and a q beyond the int32 range converts to INT_MIN or INT_MAX, which
-      // lies outside [bin_lo, nra-1) either way (nra <= 2^24)
-      const int iq = (int)q;
-      ok = a < p1 && (unsigned)(a - t.lo) < (unsigned)t.len &&
-           (unsigned)(iq - g.bin_lo) < (unsigned)(g.nra - 1 - g.bin_lo);
-      d = raw_load(col, ok ? (unsigned)iq << 3 : RAW_NO_SAMPLE);
-      double ph;
-      {
-#pragma clang fp contract(off)
-        ph = g.kphase * r + g.beta * r2;  // f64_phase's argument
-      }
-      sincos_reduce(ph, tab, e, rr);  // |ph| < 2^24 (narrow ranges); any table index is in bounds
-    };
-    auto consume = [&](bool ok, double rr, double r2, float2 d, double2 e) {
-      if (ok) {
-        double sn, cs;
-        sincos_finish(e, rr, sn, cs);
-        f64_acc(r2, sn, cs, d, ar, ai);
-        ++cnt;
-      }
-    };
-    const int64_t cstride = g.nra;
-    const float2 *colA = g.raw + (int64_t)p0 * cstride;  // pulse a's column (set A)
-    const double4 *tp = g.traj + p0;                     // tp[k]: pulse a + k
-    bool okA, okB;
-    double rA = 0.0, r2A = 0.0, rB = 0.0, r2B = 0.0;
-    float2 dA, dB;
-    double2 eA, eB;
-    issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
-    double4 t1 = tp[1], t2 = tp[2];
-    for (int a = p0; a < p1; a += 2) {  // wave-uniform pulses a (set A) and a+1 (set B)
-      issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB, eB);
-      const double4 t3 = tp[3];
-      consume(okA, rA, r2A, dA, eA);
-      colA += 2 * cstride;
-      issue(a + 2, colA, t2, okA, rA, r2A, dA, eA);
-      const double4 t4 = tp[4];
-      consume(okB, rB, r2B, dB, eB);
-      tp += 2;
-      t1 = t3;
-      t2 = t4;
-    }
+    const int c0 = min(max(p0, t.clo), p1), c1 = max(min(p1, t.chi), c0);
+    f64_narrow_span<false>(g, tab, t, p0, c0, ar, ai, cnt);
+    f64_narrow_span<true>(g, tab, t, c0, c1, ar, ai, cnt);
+    f64_narrow_span<false>(g, tab, t, c1, p1, ar, ai, cnt);
     return;
   }
   double4 tr = g.traj[p0];
@@ -257,8 +274,8 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
   // 2^21 bins, so that |f0 + dR/dra| < 2^22 for the rounding of split_bin
   s.use = s.omax <= SPLIT_OMAX_M && (double)s.omax * g.inv_dra < 2097152.0 &&
           trunc * __builtin_fabs(g.kphase) <= SPLIT_TRUNC_MAX_RAD;
-  s.core_lo = __builtin_amdgcn_readfirstlane(wave_max_i32(act ? t.lo : INT_MIN));
-  s.core_hi = __builtin_amdgcn_readfirstlane(wave_min_i32(act ? t.hi : INT_MAX));
+  s.core_lo = t.clo;
+  s.core_hi = t.chi;
   // |bin - B0| <= |f0| + |dR|/dra + 1 (an exact re-decision) with |dR| <= 1.01 |o|
   s.margin = s.use ? (int)((double)s.omax * g.inv_dra * 1.25) + 4 : INT_MAX;
   return s;
