@@ -194,11 +194,12 @@ __device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, d
                                         PulseRec &r) {  // returns B0 (absolute bin)
   const double dx = xc - t.x, dy = yc - t.y, dz = zc - t.z;
   const double r0sq = dx * dx + dy * dy + dz * dz;
-  // 1/R0 from the hardware reciprocal square root and two Newton steps (a few ulp; R0 is only the
-  // expansion point, so it needs no correct rounding -- the pixel ranges are R0 + dR exactly)
+  // 1/R0 from the hardware reciprocal square root and one Newton step (relative error about 1.5 e^2
+  // for an estimate within e; below 1e-13 for any e under 2^-22: R0 is only the expansion point, so
+  // it needs no correct rounding -- the pixel ranges are R0 + dR exactly, and an error of 1e-13 R0
+  // moves the reference bin offset by far less than the FP32 error the near-tie guard allows for)
   double inv = __builtin_amdgcn_rsq(r0sq);
   const double h = 0.5 * r0sq;
-  inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
   inv = fma(inv, fma(-h * inv, inv, 0.5), inv);
   const double r0 = r0sq * inv;
   // bin offset, kept within +-2^25 bins so that 8 x bin stays far from int32 wrap-around (a
@@ -286,11 +287,11 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 // zero).  Computed on every lane: no branch in the hot loop.
 __device__ __forceinline__ void split_mac(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
                                           float beta_t, float2 &acc) {
-  // the phase in turns: the reference's (t0, in [0, 1)) plus the linear term in one fused
-  // multiply-add, which rounds once (|c1 dR| < ~100 turns: at most 4e-6 turns, 2.4e-5 rad, and
-  // uncorrelated from pulse to pulse) and is reduced exactly, then the quadratic term
-  const float ph = fmaf(c1, dR, t0);
-  const float tt = fmaf(beta_t * dR, dR, __builtin_amdgcn_fractf(ph));
+  // the phase in turns, t0 + dR (c1 + beta' dR): the reference's (t0, in [0, 1)) plus the linear
+  // and quadratic terms in two fused multiply-adds, reduced exactly.  The inner sum rounds to the
+  // precision of c1 itself (FP32, as stored), the outer one to at most 2^-24 of |dR (c1 + beta' dR)|
+  // (< ~100 turns: 4e-6 turns, 2.4e-5 rad); both vary from pulse to pulse
+  const float tt = __builtin_amdgcn_fractf(fmaf(fmaf(beta_t, dR, c1), dR, t0));
   const float sn = __builtin_amdgcn_sinf(tt);  // sin(2 pi tt)
   const float cs = __builtin_amdgcn_cosf(tt);
   const float w = fmaf(two_r0, dR, r0sq);  // R^2 = R0^2 + 2 R0 dR (dR^2 / R0^2 < 1e-7: below FP32's resolution of R^2)
