@@ -190,6 +190,21 @@ __device__ __forceinline__ float2 raw_load(__amdgpu_buffer_rsrc_t col, unsigned 
   return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(col, (int)byte_off, 0, 0));
 }
 
+// Buffer resource of four consecutive pulse columns of the split path: col points at range bin
+// bin_lo of the first, and the range ends after bin nra-2 of the fourth (3 nra x 8 + nbin_bytes
+// bytes), so every offset it accepts lies inside RawData.  Pulse k of the four is read with the
+// scalar offset k x nra x 8, which the range check includes; the check does not stop an offset at
+// the end of its own column, so callers pass offsets of bins inside [bin_lo, nra-1) only (or one
+// beyond every column, which reads zero).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_split_group(const float2 *col, int64_t nra, unsigned nbin_bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)col, (short)0,
+                                           (int)(3 * nra * (int64_t)sizeof(float2) + nbin_bytes), 0x00020000);
+}
+
+__device__ __forceinline__ float2 raw_load_at(__amdgpu_buffer_rsrc_t r, unsigned byte_off, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)byte_off, (int)soff, 0));
+}
+
 __device__ __forceinline__ void store_mean(double2 *out, int32_t *ncnt, int64_t i, double acc_re, double acc_im,
                                            int32_t cnt) {
 #pragma clang fp contract(off)

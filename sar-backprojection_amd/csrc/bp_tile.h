@@ -400,10 +400,10 @@ template <bool FOURTH, bool FAST>
 __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, const SplitTile &s,
                                             const PulseRec *rk, int a, const float2 *col, float inv_dra,
                                             unsigned nbin_bytes, float beta_t, float2 &acc, int32_t &cnt) {
-  const int64_t nra = g.nra;  // col: bin bin_lo of the column of pulse a
-  const __amdgpu_buffer_rsrc_t c0r = raw_split_column(col, nbin_bytes), c1r = raw_split_column(col + nra, nbin_bytes),
-                               c2r = raw_split_column(col + 2 * nra, nbin_bytes),
-                               c3r = raw_split_column(col + 3 * nra, nbin_bytes);
+  // col: bin bin_lo of the column of pulse a; one resource for the four columns, pulse k at the
+  // scalar offset k x cs (every offset passed below is a bin inside the swath or beyond every column)
+  const __amdgpu_buffer_rsrc_t cr = raw_split_group(col, g.nra, nbin_bytes);
+  const unsigned cs = (unsigned)g.nra * 8u;
   SplitHalf X, Y, Z, W;
   unsigned oX, oY, oZ, oW;
   float fX, fY, fZ, fW;
@@ -430,20 +430,20 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
   }
   X.bad = Y.bad = Z.bad = W.bad = false;
   if (FAST) {
-    X.d = raw_load(c0r, oX);
-    Y.d = raw_load(c1r, oY);
-    Z.d = raw_load(c2r, oZ);
-    W.d = raw_load(c3r, oW);
+    X.d = raw_load_at(cr, oX, 0u);
+    Y.d = raw_load_at(cr, oY, cs);
+    Z.d = raw_load_at(cr, oZ, 2u * cs);
+    W.d = raw_load_at(cr, oW, 3u * cs);
   } else {
     const bool kX = oX < nbin_bytes, kY = oY < nbin_bytes, kZ = oZ < nbin_bytes, kW = oW < nbin_bytes;
     X.bad = aX && !kX;  // bin_lo <= bin < Nra-1 fails inside the aperture: not counted
     Y.bad = aY && !kY;
     Z.bad = aZ && !kZ;
     W.bad = aW && !kW;
-    X.d = raw_load(c0r, aX && kX ? oX : SPLIT_NO_SAMPLE);
-    Y.d = raw_load(c1r, aY && kY ? oY : SPLIT_NO_SAMPLE);
-    Z.d = raw_load(c2r, aZ && kZ ? oZ : SPLIT_NO_SAMPLE);
-    W.d = raw_load(c3r, aW && kW ? oW : SPLIT_NO_SAMPLE);
+    X.d = raw_load_at(cr, aX && kX ? oX : SPLIT_NO_SAMPLE, 0u);
+    Y.d = raw_load_at(cr, aY && kY ? oY : SPLIT_NO_SAMPLE, cs);
+    Z.d = raw_load_at(cr, aZ && kZ ? oZ : SPLIT_NO_SAMPLE, 2u * cs);
+    W.d = raw_load_at(cr, aW && kW ? oW : SPLIT_NO_SAMPLE, 3u * cs);
   }
   split_mac(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, acc);
   split_mac(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t, acc);
