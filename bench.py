@@ -28,11 +28,14 @@ from sarbp.metrics import split_error_report  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 BYTES_PER_PP = 8.0     # one complex64 RawData sample gathered per pixel-pulse pair
-# VALU issue quad-cycles (4 clocks) per second on MI355X: 256 CUs x 4 SIMDs at 2.4 GHz.  A wave64
-# instruction occupies one quad-cycle (FP32 sin/cos two, FP64 rsq four), and two adjacent
-# independent FP32 add/mul/fma or 32-bit add/logic instructions of one wave can issue in the same
-# quad-cycle (sarbp_probe_rates, profiles/valu_issue_rates__gfx950-probe.md).
-VALU_QUAD_PER_S = 256 * 4 * 2.4e9 / 4
+# VALU issue: a wave64 instruction occupies one quad-cycle (4 clocks) of its SIMD (FP32 sin/cos two,
+# FP64 rsq four), and two adjacent independent FP32 add/mul/fma or 32-bit add/logic instructions of
+# one wave can issue in the same quad-cycle (sarbp_probe_rates, profiles/valu_issue_rates__gfx950-probe.md).
+# MI355X has 1024 SIMDs (256 CUs x 4) in 8 XCDs; its engine clock is at most 2.4 GHz and runs lower
+# under these kernels, so the quad-cycles a launch has are taken from the clock counted in the same
+# counter pass (GRBM_GUI_ACTIVE, clocks summed over the 8 XCDs).
+SIMDS = 1024
+XCDS = 8
 PROFILE_JSON = "profiles/traffic_full-4000x250x40001.json"
 KERNEL_OF = {"split": "k_bp_split_tile<0>", "f64": "k_bp_f64_tile<false>"}
 
@@ -236,16 +239,18 @@ def main():
         extra[f"{p}_kernel_ms"] = r["kernel_s"] * 1e3
         # the same gather-bandwidth roofline for each path
         extra[f"{p}_roofline_frac"] = BYTES_PER_PP * pp / r["kernel_s"] / 1e9 / HBM_PEAK_GBS
-        # the VALU-busy fraction: quad-cycles in which the kernel's wavefronts issued VALU work per
-        # launch (rocprofv3 SQ_ACTIVE_INST_VALU, which counts each instruction's quad-cycles, less
-        # SQ_ACTIVE_INST_VALU2, the quad-cycles that issued two) over the quad-cycles 1024 SIMDs
-        # have in this launch's time
+        # the VALU-busy fraction of the profiled launch: quad-cycles in which its wavefronts issued
+        # VALU work (rocprofv3 SQ_ACTIVE_INST_VALU, which counts each instruction's quad-cycles, less
+        # SQ_ACTIVE_INST_VALU2, the quad-cycles that issued two) over the quad-cycles the 1024 SIMDs
+        # had in that launch (its GRBM_GUI_ACTIVE clocks per XCD / 4)
         kp = (prof or {}).get("kernels", {}).get(KERNEL_OF[p])
         if kp and default_maps(args):
             c = kp["counters"]
-            if "SQ_ACTIVE_INST_VALU2" in c:
+            if "SQ_ACTIVE_INST_VALU2" in c and c.get("GRBM_GUI_ACTIVE"):
                 busy = c["SQ_ACTIVE_INST_VALU"] - c["SQ_ACTIVE_INST_VALU2"]
-                extra[f"{p}_valu_busy_frac"] = busy / (VALU_QUAD_PER_S * r["kernel_s"])
+                extra[f"{p}_valu_busy_frac"] = busy / (SIMDS * c["GRBM_GUI_ACTIVE"] / XCDS / 4)
+                if kp.get("counter_dispatch_ms"):
+                    extra[f"{p}_clock_ghz"] = c["GRBM_GUI_ACTIVE"] / XCDS / (kp["counter_dispatch_ms"] * 1e-3) / 1e9
             extra[f"{p}_valu_lane_instr_per_pp"] = 64.0 * c["SQ_INSTS_VALU"] / pp
     if "split" in res and "f64" in res:
         extra["split_over_f64"] = res["split"]["pp_per_s"] / res["f64"]["pp_per_s"]
