@@ -138,13 +138,46 @@ Tiling default_tiling(int precision) {
   return t;
 }
 
-float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n) {
-  if (n <= 0) return 0.f;
-  const double *c[3] = {x, y, z};
+void PixelBoxes::add(const double *x, const double *y, const double *z, const uint8_t *mask, const int32_t *sint,
+                     int64_t i0, int64_t i1) {
+  for (int64_t i = i0; i < i1; ++i) {
+    const double p[3] = {x[i], y[i], z[i]};
+    const bool live = mask && mask[i] == 1 && sint[i] != -9999;
+    any_live = any_live || live;
+    for (int k = 0; k < 3; ++k) {
+      const double v = p[k];
+      if (!std::isfinite(v)) {
+        finite = false;
+        if (live) live_finite = false;
+        continue;
+      }
+      lo[k] = std::min(lo[k], v);
+      hi[k] = std::max(hi[k], v);
+      if (live) {
+        llo[k] = std::min(llo[k], v);
+        lhi[k] = std::max(lhi[k], v);
+      }
+    }
+  }
+}
+
+void PixelBoxes::merge(const PixelBoxes &o) {
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = std::min(lo[k], o.lo[k]);
+    hi[k] = std::max(hi[k], o.hi[k]);
+    llo[k] = std::min(llo[k], o.llo[k]);
+    lhi[k] = std::max(lhi[k], o.lhi[k]);
+  }
+  finite = finite && o.finite;
+  live_finite = live_finite && o.live_finite;
+  any_live = any_live || o.any_live;
+}
+
+float sarbp_r0min_of(const sarbp_dataset *ds, const PixelBoxes &b) {
+  if (!b.finite || !(b.lo[0] <= b.hi[0])) return 0.f;  // no pixel, or a coordinate that is not finite
   double d2 = 0.0;
   for (int k = 0; k < 3; ++k) {
-    const double lo = *std::min_element(c[k], c[k] + n), hi = *std::max_element(c[k], c[k] + n);
-    const double gap = std::max({0.0, lo - ds->thi[k], ds->tlo[k] - hi});
+    const double gap = std::max({0.0, b.lo[k] - ds->thi[k], ds->tlo[k] - b.hi[k]});
     d2 += gap * gap;
   }
   const double d = std::sqrt(d2);
@@ -152,33 +185,34 @@ float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *
   return std::nextafter((float)d, 0.f);  // rounded down
 }
 
-bool sarbp_live_range_bounds(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
-                             const uint8_t *mask, const int32_t *sint, int64_t n, double &rmin, double &rmax) {
-  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-  const double *c[3] = {x, y, z};
-  bool any = false;
-  for (int64_t i = 0; i < n; ++i) {
-    if (mask[i] != 1 || sint[i] == -9999) continue;
-    any = true;
-    for (int k = 0; k < 3; ++k) {
-      const double v = c[k][i];
-      if (!std::isfinite(v)) return false;
-      lo[k] = std::min(lo[k], v);
-      hi[k] = std::max(hi[k], v);
-    }
-  }
+float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n) {
+  if (n <= 0) return 0.f;
+  PixelBoxes b;
+  b.add(x, y, z, nullptr, nullptr, 0, n);
+  return sarbp_r0min_of(ds, b);
+}
+
+bool sarbp_range_bounds_of(const sarbp_dataset *ds, const PixelBoxes &b, double &rmin, double &rmax) {
   rmin = rmax = 0.0;
-  if (!any) return true;
+  if (!b.live_finite) return false;
+  if (!b.any_live) return true;
   double dmin = 0.0, dmax = 0.0;
   for (int k = 0; k < 3; ++k) {
-    const double gap = std::max({0.0, lo[k] - ds->thi[k], ds->tlo[k] - hi[k]});
-    const double span = std::max(hi[k] - ds->tlo[k], ds->thi[k] - lo[k]);
+    const double gap = std::max({0.0, b.llo[k] - ds->thi[k], ds->tlo[k] - b.lhi[k]});
+    const double span = std::max(b.lhi[k] - ds->tlo[k], ds->thi[k] - b.llo[k]);
     dmin += gap * gap;
     dmax += span * span;
   }
   rmin = std::sqrt(dmin) * (1.0 - 1e-12);
   rmax = std::sqrt(dmax) * (1.0 + 1e-12);
   return std::isfinite(rmax);
+}
+
+bool sarbp_live_range_bounds(const sarbp_dataset *ds, const double *x, const double *y, const double *z,
+                             const uint8_t *mask, const int32_t *sint, int64_t n, double &rmin, double &rmax) {
+  PixelBoxes b;
+  b.add(x, y, z, mask, sint, 0, n);
+  return sarbp_range_bounds_of(ds, b, rmin, rmax);
 }
 
 bool sarbp_narrow_ranges(const sarbp_dataset *ds, double r0min, double rmax) {

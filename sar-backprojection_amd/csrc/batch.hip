@@ -62,20 +62,11 @@ void DevicePool::release() {
   cached = 0;
 }
 
-void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
-                   int64_t &pp, SplitRows *rows) {
-  const int tx = r.tile_x ? r.tile_x : default_tiling(r.precision).tile_x;
-  const int64_t dimx = r.dimx > 0 ? r.dimx : (r.n > 0 ? r.n : 1);
-  const int64_t dimy = r.n > 0 ? r.n / dimx : 0;
-  tg = make_tile_grid(dimx, dimy, tx);
-  const int64_t nt = tg.ntiles();
-  tile_len.assign((size_t)nt, 0);
-  pp = 0;
-  if (rows) *rows = SplitRows();
-  if (r.n <= 0) return;
-  if (rows) rows->r0min = sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n);
+void request_tiles_band(const sarbp_dataset *ds, const sarbp_request &r, const TileGrid &tg, int64_t by0, int64_t by1,
+                        int *tile_len, int64_t &pp, SplitRows *rows) {
+  const int64_t dimx = tg.dimx, dimy = tg.dimy;
   const double *htx = ds->htx.data(), *hty = ds->hty.data(), *htz = ds->htz.data();
-  for (int64_t by = 0; by < tg.nty; ++by) {
+  for (int64_t by = by0; by < by1; ++by) {
     const int64_t y0 = by * tg.ty, y1 = std::min(dimy, y0 + tg.ty);
     for (int64_t bx = 0; bx < tg.ntx; ++bx) {
       const int64_t x0 = bx * tg.tx, x1 = std::min(dimx, x0 + tg.tx);
@@ -118,6 +109,24 @@ void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg
       tile_len[(size_t)(by * tg.ntx + bx)] = thi > tlo ? thi - tlo : 0;
     }
   }
+}
+
+TileGrid request_tile_grid(const sarbp_request &r) {
+  const int tx = r.tile_x ? r.tile_x : default_tiling(r.precision).tile_x;
+  const int64_t dimx = r.dimx > 0 ? r.dimx : (r.n > 0 ? r.n : 1);
+  const int64_t dimy = r.n > 0 ? r.n / dimx : 0;
+  return make_tile_grid(dimx, dimy, tx);
+}
+
+void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
+                   int64_t &pp, SplitRows *rows) {
+  tg = request_tile_grid(r);
+  tile_len.assign((size_t)tg.ntiles(), 0);
+  pp = 0;
+  if (rows) *rows = SplitRows();
+  if (r.n <= 0) return;
+  if (rows) rows->r0min = sarbp_pixel_r0min(ds, r.x, r.y, r.z, r.n);
+  request_tiles_band(ds, r, tg, 0, tg.nty, tile_len.data(), pp, rows);
 }
 
 void reqbuf_layout(int64_t n64, int state_slots, size_t off[10]) {

@@ -54,6 +54,13 @@ struct SplitRows {
 // the pixels tile by tile (host); with `rows`, also the tile rows' geometry above.
 void request_tiles(const sarbp_dataset *ds, const sarbp_request &r, TileGrid &tg, std::vector<int> &tile_len,
                    int64_t &pp, SplitRows *rows = nullptr);
+// The request's tile grid (its tile_x, or its precision's default).
+TileGrid request_tile_grid(const sarbp_request &r);
+// request_tiles over the tile rows [by0, by1) of tg: writes their entries of tile_len (ntiles
+// entries), adds their pixel-pulse pairs to pp and, with `rows`, folds their rows' geometry into
+// rows (any, amax, omax; not r0min).  Bands may run on different threads into separate pp and rows.
+void request_tiles_band(const sarbp_dataset *ds, const sarbp_request &r, const TileGrid &tg, int64_t by0, int64_t by1,
+                        int *tile_len, int64_t &pp, SplitRows *rows);
 // Allocate and upload one request (asynchronously on st; the caller synchronises).  state_slots:
 // BpState entries per pixel (0: none; serving: 1 for the state between rounds, or one per pulse
 // segment).  pool: where the device block comes from (null: hipMalloc); tile_len: precomputed by

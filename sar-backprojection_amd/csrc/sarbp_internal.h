@@ -1,6 +1,8 @@
 // Internal declarations shared by the HIP translation units of libsarbp.so.
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -142,9 +144,27 @@ __host__ __device__ inline double split_truncation_m(double omax, double r0min) 
 // Phase (rad) that the truncation may reach before a tile takes the FP64 terms instead.
 constexpr double SPLIT_TRUNC_MAX_RAD = 1e-4;
 
-// Lower bound of the distance from any of n pixels to any trajectory sample of ds: the distance
-// between the two bounding boxes (0 when they overlap), rounded down to float.
+// Bounding boxes of a pixel set (host): of every pixel, and of the pixels that focus (mask == 1,
+// sint != -9999), built over index ranges (in any order and split) and merged.
+struct PixelBoxes {
+  double lo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, hi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};     // every pixel
+  double llo[3] = {HUGE_VAL, HUGE_VAL, HUGE_VAL}, lhi[3] = {-HUGE_VAL, -HUGE_VAL, -HUGE_VAL};  // pixels that focus
+  bool finite = true;       // every coordinate is finite
+  bool live_finite = true;  // every coordinate of the pixels that focus is finite
+  bool any_live = false;
+  // pixels [i0, i1); mask and sint may be null (then only the box of every pixel is kept)
+  void add(const double *x, const double *y, const double *z, const uint8_t *mask, const int32_t *sint, int64_t i0,
+           int64_t i1);
+  void merge(const PixelBoxes &o);
+};
+// Lower bound of the distance from any pixel of the boxes to any trajectory sample of ds: the
+// distance between the two bounding boxes (0 when they overlap, or when a coordinate is not finite),
+// rounded down to float.
+float sarbp_r0min_of(const sarbp_dataset *ds, const PixelBoxes &b);
+// The same for n pixels.
 float sarbp_pixel_r0min(const sarbp_dataset *ds, const double *x, const double *y, const double *z, int64_t n);
+// sarbp_live_range_bounds from the boxes.
+bool sarbp_range_bounds_of(const sarbp_dataset *ds, const PixelBoxes &b, double &rmin, double &rmax);
 // Bounds [rmin, rmax] of the distances between the pixels that focus (mask == 1, sint != -9999)
 // and every trajectory sample, from the two bounding boxes (rmin rounded down, rmax up); false
 // when such a pixel has a non-finite coordinate.  No such pixel: true, rmin = rmax = 0.

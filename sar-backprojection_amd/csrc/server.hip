@@ -35,6 +35,10 @@ using Clock = std::chrono::steady_clock;
 // stores of the results reach host memory directly, visible once the round's event has completed.
 // Same size classes and reuse rule as DevicePool; its own lock (submit runs in the callers'
 // threads, finish in the scheduling and completion threads).
+// The submission pass over a large request's pixels runs on up to this many threads, about 2^16
+// pixels each.
+constexpr int SUBMIT_MAX_THREADS = 8;
+
 struct PinnedPool {
   std::mutex mu;
   std::multimap<size_t, char *> blocks;
@@ -686,21 +690,92 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   r->req = *req;  // the scheduling decisions below read the caller's arrays
   r->precision = req->precision;
   r->req.tile_x = tx;
+  // the request's page-locked block: its pixels in the device block's layout (uploaded at
+  // admission with one copy, and again after a preemption), then the means and counts, which the
+  // kernels write over PCIe as tiles finish (zero here: pixels no tile writes read as zero); its
+  // layout does not depend on the state slots, which only the device block holds
+  const int64_t nn = req->n;
+  const size_t n = (size_t)nn;
+  size_t off[10];
+  reqbuf_layout(nn, 1, off);
+  if (n) {
+    r->out_off = off[6];
+    r->host = s->hpool.get(off[8], &r->host_cap);
+    if (!r->host) return sarbp_fail(SARBP_ENOMEM, "page-locked host allocation failed");
+  }
+  auto refuse = [&](int code, const char *msg) {
+    s->hpool.put(r->host, r->host_cap);
+    r->host = nullptr;
+    return sarbp_fail(code, msg);
+  };
   int split_rows = 0;  // rows per tile of a request that runs split (its LDS records per pulse)
   {
-    TileGrid tg;
-    SplitRows rows;
+    // one pass over the pixels: tile unions, pixel-pulse pairs, the split rows' geometry, the
+    // bounding boxes and the copy into the page-locked block, in bands of tile rows; a large request
+    // splits the bands over threads of its own (about 2^16 pixels each)
     const bool split = req->precision == SARBP_PREC_SPLIT;
-    request_tiles(ds, r->req, tg, r->tile_len, r->pp, split ? &rows : nullptr);
-    r->narrow = request_narrow(ds, r->req);
+    const TileGrid tg = request_tile_grid(r->req);
+    r->tile_len.assign((size_t)tg.ntiles(), 0);
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)SUBMIT_MAX_THREADS, nn >> 16, tg.nty}));
+    std::vector<int64_t> pps((size_t)parts, 0);
+    std::vector<SplitRows> rws((size_t)parts);
+    std::vector<PixelBoxes> boxes((size_t)parts);
+    char *h = r->host;
+    auto band = [&](int k) {
+      const int64_t by0 = tg.nty * k / parts, by1 = tg.nty * (k + 1) / parts;
+      if (!n) return;
+      request_tiles_band(ds, r->req, tg, by0, by1, r->tile_len.data(), pps[(size_t)k], split ? &rws[(size_t)k] : nullptr);
+      const size_t i0 = (size_t)std::min<int64_t>(nn, by0 * tg.ty * tg.dimx);
+      const size_t i1 = (size_t)std::min<int64_t>(nn, by1 * tg.ty * tg.dimx);
+      boxes[(size_t)k].add(req->x, req->y, req->z, req->mask, req->sint, (int64_t)i0, (int64_t)i1);
+      const size_t m = i1 - i0;
+      std::memcpy(h + off[0] + i0 * 8, req->x + i0, m * 8);
+      std::memcpy(h + off[1] + i0 * 8, req->y + i0, m * 8);
+      std::memcpy(h + off[2] + i0 * 8, req->z + i0, m * 8);
+      std::memcpy(h + off[3] + i0, req->mask + i0, m);
+      std::memcpy(h + off[4] + i0 * 4, req->index + i0, m * 4);
+      std::memcpy(h + off[5] + i0 * 4, req->sint + i0, m * 4);
+      std::memset(h + off[6] + i0 * 16, 0, m * 16);
+      std::memset(h + off[7] + i0 * 4, 0, m * 4);
+      if (k == 0) {  // the alignment gaps after the means and the counts
+        std::memset(h + off[6] + n * 16, 0, off[7] - off[6] - n * 16);
+        std::memset(h + off[7] + n * 4, 0, off[8] - off[7] - n * 4);
+      }
+    };
+    {
+      std::vector<std::thread> th;
+      int k = 1;
+      try {
+        for (; k < parts; ++k) th.emplace_back(band, k);
+      } catch (...) {  // no thread to be had: the rest of the bands in this one
+      }
+      for (int j = k; j < parts; ++j) band(j);
+      band(0);
+      for (std::thread &t : th) t.join();
+    }
+    r->pp = 0;
+    SplitRows rows;
+    PixelBoxes box;
+    for (int k = 0; k < parts; ++k) {
+      r->pp += pps[(size_t)k];
+      const SplitRows &q = rws[(size_t)k];
+      rows.any = rows.any || q.any;
+      rows.amax = std::max(rows.amax, q.amax);
+      rows.omax = std::max(rows.omax, q.omax);
+      box.merge(boxes[(size_t)k]);
+    }
+    double rmin = 0.0, rmax = 0.0;
+    r->narrow = nn <= 0 || (sarbp_range_bounds_of(ds, box, rmin, rmax) && sarbp_narrow_ranges(ds, rmin, rmax));
     if (split) {
+      rows.r0min = nn > 0 ? sarbp_r0min_of(ds, box) : 0.0;
       r->est = split_phase_estimate(ds, rows);
       r->r0min = (float)rows.r0min;
       if (r->est > s->cfg.phase_err_limit_rad) {  // FP64 terms, on the FP64 tiling
         r->precision = SARBP_PREC_F64;
         r->req.precision = SARBP_PREC_F64;
         r->req.tile_x = 0;
-        request_tiles(ds, r->req, tg, r->tile_len, r->pp);
+        TileGrid tg64;
+        request_tiles(ds, r->req, tg64, r->tile_len, r->pp);
       } else {
         split_rows = tg.ty;
       }
@@ -709,8 +784,8 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   // a split request's records of one pulse chunk must fit the batch kernel's LDS; checked here so
   // that the request is refused alone instead of failing the round it would join
   if (split_rows > 0 && sarbp_batch_lds_bytes(1, s->cfg.pulse_chunk, split_rows) > SARBP_BATCH_LDS_MAX)
-    return sarbp_fail(SARBP_EINVAL, "split request: pulse_chunk x tile rows x 48 B exceeds the batch kernel's "
-                                    "64 KiB of LDS (use a wider tile_x or a smaller pulse_chunk)");
+    return refuse(SARBP_EINVAL, "split request: pulse_chunk x tile rows x 48 B exceeds the batch kernel's "
+                                "64 KiB of LDS (use a wider tile_x or a smaller pulse_chunk)");
   {
     int max_len = 0;
     int64_t live = 0;
@@ -724,28 +799,11 @@ int sarbp_server_submit(sarbp_server *s, sarbp_dataset *ds, const sarbp_request 
   // device bytes while admitted: pixels + outputs + state or segment slots (each array 256-byte
   // aligned)
   auto up = [](int64_t v) { return (v + 255) / 256 * 256; };
-  const int64_t nn = req->n;
   r->need = up(nn * 8) * 3 + up(nn) + up(nn * 4) * 2 + up(nn * 16) + up(nn * 4) +
             up(nn * (int64_t)sizeof(BpState) * r->nseg);
-  if (r->need > s->cfg.hbm_budget_bytes) return sarbp_fail(SARBP_EBUDGET, "request exceeds the device-memory budget");
-  // the request's page-locked block: its pixels in the device block's layout (uploaded at
-  // admission with one copy, and again after a preemption), then the means and counts, which the
-  // kernels write over PCIe as tiles finish (zero here: pixels no tile writes read as zero)
-  const size_t n = (size_t)nn;
+  if (r->need > s->cfg.hbm_budget_bytes) return refuse(SARBP_EBUDGET, "request exceeds the device-memory budget");
   if (n) {
-    size_t off[10];
-    reqbuf_layout(nn, r->nseg, off);
-    r->out_off = off[6];
-    r->host = s->hpool.get(off[8], &r->host_cap);
-    if (!r->host) return sarbp_fail(SARBP_ENOMEM, "page-locked host allocation failed");
     char *h = r->host;
-    std::memcpy(h + off[0], req->x, n * 8);
-    std::memcpy(h + off[1], req->y, n * 8);
-    std::memcpy(h + off[2], req->z, n * 8);
-    std::memcpy(h + off[3], req->mask, n);
-    std::memcpy(h + off[4], req->index, n * 4);
-    std::memcpy(h + off[5], req->sint, n * 4);
-    std::memset(h + off[6], 0, off[8] - off[6]);
     r->req.x = (const double *)(h + off[0]);
     r->req.y = (const double *)(h + off[1]);
     r->req.z = (const double *)(h + off[2]);

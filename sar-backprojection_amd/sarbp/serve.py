@@ -162,8 +162,8 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
     """Submit requests drawn from ``pool`` (list of (dataset index, Request)) at the times of a seeded
     Poisson schedule of rate ``rate_hz`` for ``duration_s``; wait for all.
 
-    ``submit_threads`` driver threads take the scheduled arrivals in order, each sleeping until its
-    next arrival and submitting it (the library copies the pixels in the caller's thread, so one
+    ``submit_threads`` driver threads take the scheduled arrivals in order, each sleeping until just
+    before its next arrival, yielding until the arrival and submitting it (the library copies the pixels in the caller's thread, so one
     thread caps the rate a driver can offer).  Latency is completion minus ARRIVAL, the request's
     time in the Poisson schedule: it includes any delay before the driver got to submit it (a driver
     that falls behind the schedule shows up as latency), the submission itself and the queueing.
@@ -194,9 +194,12 @@ def poisson_run(server: Server, datasets, pool, rate_hz: float, duration_s: floa
                 nxt[0] += 1
             if j >= len(arrivals):
                 return
-            wait = t0 + arrivals[j] - time.perf_counter()
-            if wait > 0:
-                time.sleep(wait)
+            t_arr = t0 + arrivals[j]
+            wait = t_arr - time.perf_counter()
+            if wait > 0.002:
+                time.sleep(wait - 0.0015)  # a sleeping thread wakes up to ~1 ms late
+            while time.perf_counter() < t_arr:
+                time.sleep(0)  # the last stretch in yields, which release the interpreter lock
             di, req = pool[j % len(pool)]
             t_call = time.perf_counter()
             try:
@@ -272,7 +275,7 @@ def size_sweep(server: Server, ds, scene, sizes=((8, 8), (128, 64), (256, 128), 
     """Latency of isolated requests by size: each (width, height) patch of ``scene`` at (x0, y0) is
     submitted ``reps`` times, one at a time; the last repetition's phases are reported (ms):
     submission to admission, admission to its round, the round (work list, kernels, result copies),
-    and the whole latency."""
+    and the whole latency, with every repetition's latency and round."""
     rows = []
     nx, ny = scene.shape
     for (w, h) in sizes:
@@ -282,15 +285,19 @@ def size_sweep(server: Server, ds, scene, sizes=((8, 8), (128, 64), (256, 128), 
         for prec in precisions:
             req = Request(f(sub.x), f(sub.y), f(sub.z), f(sub.mask), f(sub.index), f(sub.sint), dimx=w,
                           precision=prec)
+            infos = []
             for _ in range(reps):
                 hd = server.submit(ds, req)
                 hd.wait()
-                i = hd.info()
+                infos.append(hd.info())
                 time.sleep(0.002)
-            ms = lambda a, b: (i[b] - i[a]) * 1e3  # noqa: E731
+            i = infos[-1]
+            ms = lambda a, b, i=i: (i[b] - i[a]) * 1e3  # noqa: E731
             rows.append({"w": w, "h": h, "precision": prec, "pp": i["pp"], "segments": i["segments"],
                          "submit_to_admit_ms": ms("submit_s", "admit_s"), "admit_to_round_ms": ms("admit_s", "round_s"),
-                         "round_ms": ms("round_s", "gpu_done_s"), "latency_ms": ms("submit_s", "done_s")})
+                         "round_ms": ms("round_s", "gpu_done_s"), "latency_ms": ms("submit_s", "done_s"),
+                         "latency_ms_each": [(q["done_s"] - q["submit_s"]) * 1e3 for q in infos],
+                         "round_ms_each": [(q["gpu_done_s"] - q["round_s"]) * 1e3 for q in infos]})
     return rows
 
 
@@ -319,6 +326,7 @@ def main(argv=None):
     ap.add_argument("--submit-threads", type=int, default=4, help="driver threads submitting the arrivals")
     ap.add_argument("--small", action="store_true")
     ap.add_argument("--sweep", action="store_true", help="latency of isolated requests by size instead")
+    ap.add_argument("--sweep-reps", type=int, default=4, help="repetitions of each request of the size sweep")
     a = ap.parse_args(argv)
     from . import synth
     from .core import Dataset
@@ -328,7 +336,7 @@ def main(argv=None):
     if a.sweep:
         ds = Dataset(base.raw, base.tx, base.ty, base.tz, base.near, base.dra, base.lam)
         srv = Server(hbm_budget_bytes=int(a.budget_gb * 1e9), quantum_ms=a.quantum_ms, max_segments=a.max_segments)
-        rows = size_sweep(srv, ds, base, x0=min(1000, base.shape[0] // 4))
+        rows = size_sweep(srv, ds, base, x0=min(1000, base.shape[0] // 4), reps=a.sweep_reps)
         srv.close()
         ds.close()
         for r in rows:
