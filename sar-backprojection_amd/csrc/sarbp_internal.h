@@ -103,7 +103,7 @@ struct sarbp_scene {
 
 // Samples the device trajectory holds past pulse naz-1 (copies of the last one), so that kernels
 // may read a few pulses ahead of a loop's end without clamping the index.
-constexpr int SARBP_TRAJ_PAD = 4;
+constexpr int SARBP_TRAJ_PAD = 8;
 
 // Aperture of one pixel (sar_focuser_puntual.py:57-66): J = 2*ceil(s/2) + 1 pulses starting at
 // index - ceil(|s|/2); integer forms of the reference's float ceil on an int32.
