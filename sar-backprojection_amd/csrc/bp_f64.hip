@@ -54,7 +54,7 @@ __global__ __launch_bounds__(64) void k_bp_f64_tile(BpGeom g, PixelsView px, Til
   const TileLane t = tile_lane(g, px, tg, tile, threadIdx.x);
   double ar = 0.0, ai = 0.0;
   int32_t cnt = 0;
-  f64_span<WIDE>(g, tab, t, t.wlo, t.whi, ar, ai, cnt);
+  f64_span<WIDE, 3>(g, tab, t, t.wlo, t.whi, ar, ai, cnt);
   if (t.valid) store_mean(out, ncnt, t.i, ar, ai, cnt);
 }
 

@@ -74,16 +74,18 @@ __device__ __forceinline__ TileLane tile_lane(const BpGeom &g, const PixelsView 
 }
 
 // FP64 terms of the narrow-range variant for pulses [p0, p1) of the union (tab: the sin/cos table),
-// software-pipelined three pulses deep: pulse a+2's range, bin and gather are issued before pulse
-// a's phase, sin/cos and accumulation, so the gather's latency is covered by two pulses of
-// arithmetic and each wave has three independent chains of FP64 work in flight (three-deep runs 2 %
-// faster at seven wavefronts per SIMD than two-deep at eight; measured).  CORE: [p0, p1) lies inside every non-empty aperture of the tile, so a lane tests only
+// software-pipelined DEPTH (2 or 3) pulses deep: pulse a+DEPTH-1's range, bin and gather are issued
+// before pulse a's phase, sin/cos and accumulation, so the gather's latency is covered by DEPTH-1
+// pulses of arithmetic and each wave has DEPTH independent chains of FP64 work in flight.  The
+// scene kernel runs three deep (2 % faster at seven wavefronts per SIMD than two deep at eight;
+// measured); the batch kernels, whose register budget also holds the saved state and the other
+// precision's path, two deep (three deep spills in their loops).  CORE: [p0, p1) lies inside every non-empty aperture of the tile, so a lane tests only
 // that it has an aperture, not where the pulse lies in it.
-template <bool CORE>
+template <bool CORE, int DEPTH>
 __device__ __forceinline__ void f64_narrow_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0,
                                                 int p1, double &ar, double &ai, int32_t &cnt) {
   if (p0 >= p1) return;
-  // three register sets, A, B and C, rotate: while two pulses' gathers are in flight the third
+  // DEPTH register sets (A, B[, C]) rotate: while the other pulses' gathers are in flight one
   // pulse's phase and sums are computed.  Column and trajectory pointers advance by addition
   // (the device trajectory holds SARBP_TRAJ_PAD samples past the last pulse, so reading up to
   // seven pulses ahead needs no clamp; a column past the last pulse is only addressed with
@@ -130,24 +132,41 @@ __device__ __forceinline__ void f64_narrow_span(const BpGeom &g, const double2 *
   double rA = 0.0, r2A = 0.0, rB = 0.0, r2B = 0.0, rC = 0.0, r2C = 0.0;
   float2 dA, dB, dC;
   double2 eA, eB, eC;
-  issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
-  issue(p0 + 1, colA + cstride, tp[1], okB, rB, r2B, dB, eB);
-  double4 t2 = tp[2], t3 = tp[3], t4 = tp[4];
-  for (int a = p0; a < p1; a += 3) {  // wave-uniform pulses a, a+1, a+2 (sets A, B, C)
-    issue(a + 2, colA + 2 * cstride, t2, okC, rC, r2C, dC, eC);
-    const double4 t5 = tp[5];
-    consume(okA, rA, r2A, dA, eA);
-    issue(a + 3, colA + 3 * cstride, t3, okA, rA, r2A, dA, eA);
-    const double4 t6 = tp[6];
-    consume(okB, rB, r2B, dB, eB);
-    issue(a + 4, colA + 4 * cstride, t4, okB, rB, r2B, dB, eB);
-    const double4 t7 = tp[7];
-    consume(okC, rC, r2C, dC, eC);
-    colA += 3 * cstride;
-    tp += 3;
-    t2 = t5;
-    t3 = t6;
-    t4 = t7;
+  if constexpr (DEPTH == 2) {
+    issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
+    double4 t1 = tp[1], t2 = tp[2];
+    for (int a = p0; a < p1; a += 2) {  // wave-uniform pulses a (set A) and a+1 (set B)
+      issue(a + 1, colA + cstride, t1, okB, rB, r2B, dB, eB);
+      const double4 t3 = tp[3];
+      consume(okA, rA, r2A, dA, eA);
+      colA += 2 * cstride;
+      issue(a + 2, colA, t2, okA, rA, r2A, dA, eA);
+      const double4 t4 = tp[4];
+      consume(okB, rB, r2B, dB, eB);
+      tp += 2;
+      t1 = t3;
+      t2 = t4;
+    }
+  } else {
+    issue(p0, colA, tp[0], okA, rA, r2A, dA, eA);
+    issue(p0 + 1, colA + cstride, tp[1], okB, rB, r2B, dB, eB);
+    double4 t2 = tp[2], t3 = tp[3], t4 = tp[4];
+    for (int a = p0; a < p1; a += 3) {  // wave-uniform pulses a, a+1, a+2 (sets A, B, C)
+      issue(a + 2, colA + 2 * cstride, t2, okC, rC, r2C, dC, eC);
+      const double4 t5 = tp[5];
+      consume(okA, rA, r2A, dA, eA);
+      issue(a + 3, colA + 3 * cstride, t3, okA, rA, r2A, dA, eA);
+      const double4 t6 = tp[6];
+      consume(okB, rB, r2B, dB, eB);
+      issue(a + 4, colA + 4 * cstride, t4, okB, rB, r2B, dB, eB);
+      const double4 t7 = tp[7];
+      consume(okC, rC, r2C, dC, eC);
+      colA += 3 * cstride;
+      tp += 3;
+      t2 = t5;
+      t3 = t6;
+      t4 = t7;
+    }
   }
 }
 
@@ -157,15 +176,15 @@ __device__ __forceinline__ void f64_narrow_span(const BpGeom &g, const double2 *
 // WIDE = false (the narrow-range variant, which has registers to spare) takes f64_narrow_span, the
 // pulses inside every aperture of the tile without the per-lane aperture test; the terms and their
 // order into the sums are the same.
-template <bool WIDE = true>
+template <bool WIDE = true, int DEPTH = 2>
 __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, const TileLane &t, int p0, int p1,
                                          double &ar, double &ai, int32_t &cnt) {
   if (p0 >= p1) return;
   if constexpr (!WIDE) {
     const int c0 = min(max(p0, t.clo), p1), c1 = max(min(p1, t.chi), c0);
-    f64_narrow_span<false>(g, tab, t, p0, c0, ar, ai, cnt);
-    f64_narrow_span<true>(g, tab, t, c0, c1, ar, ai, cnt);
-    f64_narrow_span<false>(g, tab, t, c1, p1, ar, ai, cnt);
+    f64_narrow_span<false, DEPTH>(g, tab, t, p0, c0, ar, ai, cnt);
+    f64_narrow_span<true, DEPTH>(g, tab, t, c0, c1, ar, ai, cnt);
+    f64_narrow_span<false, DEPTH>(g, tab, t, c1, p1, ar, ai, cnt);
     return;
   }
   double4 tr = g.traj[p0];
