@@ -103,7 +103,8 @@ struct ItemBuffers {
   int32_t *d_seg = nullptr, *d_heads = nullptr;
   int32_t *d_list = nullptr;  // indices into d_reqs (the serving loop's segmented requests)
   size_t cap_reqs = 0, cap_items = 0, cap_list = 0;
-  char *h_stage = nullptr;
+  char *h_stage = nullptr;        // page-locked staging block of the work list
+  const char *d_stage = nullptr;  // the same block as the device addresses it
   size_t cap_stage = 0;
   int nitems = 0;
   int upload(const std::vector<BatchReqDev> &reqs, const std::vector<BatchItem> &items,
