@@ -207,9 +207,10 @@ __device__ __forceinline__ void f64_span(const BpGeom &g, const double2 *tab, co
 constexpr float SPLIT_ROUND_MAGIC = 12582912.0f;
 
 struct PulseRec {  // one pulse's tile-relative coefficients (3 x 16 B LDS reads)
-  float4 a;        // ux, uy, uz, 1/(2R0)
-  float4 b;        // 1/(2R0^2), f0 = q0 - B0, int bits: byte offset of bin B0 less 8 bits(magic), t0 = frac(phi0/2pi)
-  float4 c;        // c1 = (k + 2 beta R0)/2pi, R0^2, 2R0, 1/(8R0^3)
+  float4 a;        // ux, uy, uz, 1/(2R0)                                          (range offset)
+  float4 b;        // 1/(2R0^2), f0 = q0 - B0, int bits: byte offset of bin B0 less 8 bits(magic),
+                   // 1/(8R0^3)                                                      (bin)
+  float4 c;        // t0 = frac(phi0/2pi), c1 = (k + 2 beta R0)/2pi, R0^2, 2R0       (phase and weight)
 };
 
 // Dynamic LDS of every kernel that stages split records (indexed, so addressing stays 32-bit).
@@ -239,8 +240,8 @@ __device__ __forceinline__ int make_rec(const BpGeom &g, double xc, double yc, d
   // byte offset of bin B0 from bin bin_lo, where the split path's column resources start
   r.b = make_float4((float)(0.5 * inv * inv), (float)(q0 - b0),
                     __uint_as_float((unsigned)((int)b0 - g.bin_lo) * 8u - 8u * (unsigned)__float_as_int(SPLIT_ROUND_MAGIC)),
-                    (float)t0);
-  r.c = make_float4((float)c1, (float)r0sq, (float)(2.0 * r0), (float)(0.125 * inv * inv * inv));
+                    (float)(0.125 * inv * inv * inv));
+  r.c = make_float4((float)t0, (float)c1, (float)r0sq, (float)(2.0 * r0));
   return (int)b0;
 }
 
@@ -309,9 +310,12 @@ __device__ __forceinline__ SplitTile split_tile(const BpGeom &g, const TileGrid 
 
 // Second half of one split term: phase, sin/cos and the weighted term R^2 d exp(-j phi), added to
 // acc by four fused multiply-adds (adds nothing when the pulse does not contribute: its sample d is
-// zero).  Computed on every lane: no branch in the hot loop.
-__device__ __forceinline__ void split_mac(float dR, float2 d, float t0, float c1, float r0sq, float two_r0,
-                                          float beta_t, float2 &acc) {
+// zero).  Computed on every lane: no branch in the hot loop.  The phase and weight coefficients are
+// read from the record here, after the gather, so that only dR and the sample stay live while the
+// gather is in flight.
+__device__ __forceinline__ void split_mac(float dR, float2 d, const PulseRec *rk, float beta_t, float2 &acc) {
+  const float4 rc = rk->c;
+  const float t0 = rc.x, c1 = rc.y, r0sq = rc.z, two_r0 = rc.w;
   // the phase in turns, t0 + dR (c1 + beta' dR): the reference's (t0, in [0, 1)) plus the linear
   // and quadratic terms in two fused multiply-adds, reduced exactly.  The inner sum rounds to the
   // precision of c1 itself (FP32, as stored), the outer one to at most 2^-24 of |dR (c1 + beta' dR)|
@@ -348,7 +352,7 @@ __device__ __forceinline__ void split_acc(float2 t, float &sr, float &si, float 
 // into a zero sample without a memory access.
 struct SplitHalf {
   bool bad;  // inside the aperture but the bin is outside [0, Nra-1): not counted (rare)
-  float dR, t0, c1, r0sq, two_r0;
+  float dR;
   float2 d;
 };
 
@@ -361,17 +365,13 @@ constexpr unsigned SPLIT_NO_SAMPLE = RAW_NO_SAMPLE;
 template <bool FOURTH>
 __device__ __forceinline__ void split_bin(const SplitTile &s, const PulseRec *rk, float inv_dra, SplitHalf &h,
                                           unsigned &off, float &fr) {
-  const float4 ra = rk->a, rb = rk->b, rc = rk->c;
-  h.t0 = rb.w;
-  h.c1 = rc.x;
-  h.r0sq = rc.y;
-  h.two_r0 = rc.z;
+  const float4 ra = rk->a, rb = rk->b;
   // computed on every lane (lanes outside their aperture have zero offsets: finite, unused)
   const float av = fmaf(ra.z, s.oz, fmaf(ra.y, s.oy, ra.x * s.ox));
   const float t2 = fmaf(-av, av, s.qo);
   // dR = a + t2 (1/(2R0) - a/(2R0^2) [- (|o|^2 - 5a^2)/(8R0^3)])
   float c = fmaf(-av, rb.x, ra.w);
-  if (FOURTH) c = fmaf(-fmaf(-5.f * av, av, s.qo), rc.w, c);
+  if (FOURTH) c = fmaf(-fmaf(-5.f * av, av, s.qo), rb.w, c);
   const float dR = fmaf(t2, c, av);
   h.dR = dR;
   const float qb = fmaf(dR, inv_dra, rb.y);
@@ -470,10 +470,10 @@ __device__ __forceinline__ void split_group(const BpGeom &g, const TileLane &t, 
     Z.d = raw_load_at(cr, aZ && kZ ? oZ : SPLIT_NO_SAMPLE, 2u * cs);
     W.d = raw_load_at(cr, aW && kW ? oW : SPLIT_NO_SAMPLE, 3u * cs);
   }
-  split_mac(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, acc);
-  split_mac(Y.dR, Y.d, Y.t0, Y.c1, Y.r0sq, Y.two_r0, beta_t, acc);
-  split_mac(Z.dR, Z.d, Z.t0, Z.c1, Z.r0sq, Z.two_r0, beta_t, acc);
-  split_mac(W.dR, W.d, W.t0, W.c1, W.r0sq, W.two_r0, beta_t, acc);
+  split_mac(X.dR, X.d, rk, beta_t, acc);
+  split_mac(Y.dR, Y.d, rk + 1, beta_t, acc);
+  split_mac(Z.dR, Z.d, rk + 2, beta_t, acc);
+  split_mac(W.dR, W.d, rk + 3, beta_t, acc);
   if (!FAST && (X.bad || Y.bad || Z.bad || W.bad)) {  // rare: a bin outside the swath
     __asm__ volatile("");  // keeps the branch (the count is not worth computing on every group)
     cnt -= (X.bad ? 1 : 0) + (Y.bad ? 1 : 0) + (Z.bad ? 1 : 0) + (W.bad ? 1 : 0);
@@ -559,7 +559,7 @@ __device__ __forceinline__ void split_span_t(const BpGeom &g, const TileLane &t,
       SplitHalf X;
       split_first<FOURTH>(g, t, s, rk, c0 + k, raw_split_column(col, nbin_bytes), inv_dra, nbin_bytes, X);
       float2 acc = make_float2(0.f, 0.f);
-      split_mac(X.dR, X.d, X.t0, X.c1, X.r0sq, X.two_r0, beta_t, acc);
+      split_mac(X.dR, X.d, rk, beta_t, acc);
       split_acc<ACCUM>(acc, sr, si, cr, ci);
       cnt -= X.bad ? 1 : 0;
     }
