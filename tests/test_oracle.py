@@ -25,7 +25,8 @@ BETA = 4 * PI32 * ALFA / (C * C)
 
 
 def test_constants_match_reference_values():
-    # float32(pi) held in a double, and the beta the survey of the reference states
+    # float32(pi) held in a double, and beta = 4 pi alfa / c^2 from the reference's constants
+    # (sar_focuser_puntual.py:35-40) evaluated in FP64
     assert float(np.float32(np.pi)) == 3.141592741012573242
     assert abs(B.BETA_F64 - 4.664368939847232e-05) < 1e-20
 
